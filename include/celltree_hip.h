@@ -1,0 +1,223 @@
+/*
+ * celltree_hip.h -- C ABI of libcelltree_hip.so, the MI355X (gfx950) cell-tree library.
+ *
+ * Plain C: fixed-width integers, plain pointers and sizes, no HIP or torch types.
+ * Conventions:
+ *   - every array pointer is a DEVICE pointer unless its comment says "host";
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream); every call is
+ *     stream-ordered and asynchronous unless its comment says it synchronizes;
+ *   - every function returns CT_OK (0) or a negative CT_ERR_* code and never throws;
+ *     ct_last_error() returns the message of the calling thread's last failure;
+ *   - result index arrays are int64 (numpy's intp, the reference's IntDType);
+ *   - element arrays (faces, edges) are int32 on the device, padded with -1.
+ *
+ * Each declaration names the numba_celltree v0.4.2 function it replaces (file:line,
+ * relative to the numba_celltree source tree).
+ */
+#ifndef CELLTREE_HIP_H
+#define CELLTREE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CT_OK 0
+#define CT_ERR_ARG (-1)        /* invalid argument (shape, size, null pointer) */
+#define CT_ERR_HIP (-2)        /* a HIP runtime call failed */
+#define CT_ERR_DEGENERATE (-3) /* tree build: a centroid falls in no bucket (the reference raises IndexError) */
+#define CT_ERR_DEPTH (-4)      /* tree deeper than the largest traversal stack compiled in */
+#define CT_ERR_WORKSPACE (-5)  /* workspace smaller than the matching *_workspace_bytes() */
+
+#define CT_MAX_N_VERTEX 32     /* constants.py:134 */
+#define CT_MAX_DEPTH 254       /* deepest tree the traversal kernels accept */
+
+/* One tree node, 32 bytes, the device form of NodeDType (constants.py:92-106).
+ * child == -1 marks a leaf; the right child is child + 1. */
+typedef struct ct_node {
+  double Lmax;
+  double Rmin;
+  int32_t child;
+  int32_t ptr;
+  int32_t size;
+  int32_t dim;
+} ct_node;
+
+/* A built tree: the device form of CellTreeData (constants.py:82-89). */
+typedef struct ct_tree {
+  const double* vertices;    /* (n_vertices, 2) */
+  const int32_t* elements;   /* (n_elements, n_max_vert), faces (CCW) or edges (n_max_vert 2) */
+  const double* bb_coords;   /* (n_elements, 4): xmin, xmax, ymin, ymax */
+  const ct_node* nodes;      /* (n_nodes) */
+  const int32_t* bb_indices; /* (n_elements) */
+  int64_t n_vertices;
+  int64_t n_elements;
+  int64_t n_nodes;
+  int32_t n_max_vert;
+  int32_t depth;             /* longest root-to-leaf path, in edges */
+  double bbox[4];            /* host values: tree bounding box */
+} ct_tree;
+
+const char* ct_last_error(void);
+int ct_version(void);
+
+/* ---------------------------------------------------------------- geometry helpers */
+
+/* geometry_utils.py:541-561 counter_clockwise: reorders clockwise rows in place. */
+int ct_counter_clockwise(const double* vertices, int32_t* faces, int64_t n_faces,
+                         int32_t n_max_vert, void* stream);
+
+/* geometry_utils.py:443-456 build_face_bboxes -> out (n_faces, 4). */
+int ct_build_face_bboxes(const int32_t* faces, int64_t n_faces, int32_t n_max_vert,
+                         const double* vertices, double* out, void* stream);
+
+/* geometry_utils.py:473-487 build_edge_bboxes (padded by tol) -> out (n_edges, 4). */
+int ct_build_edge_bboxes(const int32_t* edges, int64_t n_edges, const double* vertices,
+                         double tol, double* out, void* stream);
+
+/* celltree_base.py:20-48 bbox_tree + bbox_distances.  distances: (n, 3) dx, dy, dxy;
+ * out5: device array of 5 doubles = xmin, xmax, ymin, ymax, max(dxy) (the argument of
+ * default_tolerance, celltree_base.py:51-52). */
+int ct_bbox_stats(const double* bb_coords, int64_t n, double* distances, double* out5,
+                  void* stream);
+
+/* edge_celltree.py:60-64: out4 = xmin, xmax, ymin, ymax of the vertices. */
+int ct_vertex_extent(const double* vertices, int64_t n, double* out4, void* stream);
+
+/* ------------------------------------------------------------------------- build */
+
+/* creation.py:216-230 pessimistic_n_nodes: capacity of the nodes_out array of ct_build. */
+int64_t ct_max_nodes(int64_t n_elements);
+
+/* Device workspace, in bytes, that ct_build needs for n_elements and n_buckets. */
+size_t ct_build_workspace_bytes(int64_t n_elements, int32_t n_buckets);
+
+/* creation.py:384-413 initialize (+ build, :233-381): level-synchronous bucket-split
+ * build on the device.  Writes nodes_out[0 .. *n_nodes) in the reference's DFS numbering
+ * and bb_indices_out (n_elements).  n_nodes and depth are HOST outputs; the call
+ * synchronizes the stream once per tree level. */
+int ct_build(const double* bb_coords, int64_t n_elements, int32_t n_buckets,
+             int32_t cells_per_leaf, void* workspace, size_t workspace_bytes,
+             ct_node* nodes_out, int32_t* bb_indices_out, int64_t* n_nodes, int32_t* depth,
+             void* stream);
+
+/* query.py:568-622 collect_node_bounds -> out (n_nodes, 4); bbox is a HOST array of 4;
+ * parent_scratch: device int32 array of n_nodes. */
+int ct_collect_node_bounds(const ct_node* nodes, int64_t n_nodes, const double* bbox,
+                           int32_t* parent_scratch, double* out, void* stream);
+
+/* query.py:625-664 validate_node_bounds -> validity (n_nodes) of 0/1 bytes. */
+int ct_validate_node_bounds(const ct_tree* tree, const double* node_bounds, uint8_t* validity,
+                            void* stream);
+
+/* ----------------------------------------------------------------------- queries */
+
+/* query.py:110-117 locate_points: first containing face in DFS order, else -1. */
+int ct_locate_points(const ct_tree* tree, const double* points, int64_t n, double tol,
+                     int64_t* out, void* stream);
+
+/* query.py:167-174 locate_points_on_edge (EdgeCellTree2d). */
+int ct_locate_points_on_edge(const ct_tree* tree, const double* points, int64_t n, double tol,
+                             int64_t* out, void* stream);
+
+/* query.py:278-289 locate_boxes, first pass: counts[i] = number of tree cells whose bbox
+ * strictly overlaps box i. */
+int ct_locate_boxes_count(const ct_tree* tree, const double* boxes, int64_t n, int64_t* counts,
+                          void* stream);
+
+/* query.py:278-289 locate_boxes, second pass: writes pairs at offsets[i] (from
+ * ct_exclusive_scan), in box order then per-box DFS order (the reference's order). */
+int ct_locate_boxes_write(const ct_tree* tree, const double* boxes, int64_t n,
+                          const int64_t* offsets, int64_t* ii, int64_t* jj, void* stream);
+
+/* query.py:500-544 locate_edge_faces (kind 1) / locate_edge_edges (kind 0), first pass. */
+int ct_locate_edges_count(const ct_tree* tree, const double* segments, int64_t n, int32_t kind,
+                          int64_t* counts, void* stream);
+
+/* Second pass: ii, jj and xy (n_found, 2, 2) in segment order then DFS order. */
+int ct_locate_edges_write(const ct_tree* tree, const double* segments, int64_t n, int32_t kind,
+                          const int64_t* offsets, int64_t* ii, int64_t* jj, double* xy,
+                          void* stream);
+
+/* geometry_utils.py:564-574 sort_intersections_by_edge on grouped output: within each
+ * segment's range [offsets[i], offsets[i+1]) stably sorts jj and xy by
+ * t = (xy[:,0] - a) . (b - a), NaN last, in place. */
+int ct_sort_intersections_by_edge(const double* segments, int64_t n, const int64_t* offsets,
+                                  int64_t* jj, double* xy, void* stream);
+
+/* ------------------------------------------------------------- scan / compaction */
+
+size_t ct_scan_workspace_bytes(int64_t n);
+
+/* out[0] = 0, out[i+1] = out[i] + in[i]: exclusive offsets plus the total in out[n]. */
+int ct_exclusive_scan(const int64_t* in, int64_t n, int64_t* out, void* workspace,
+                      size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------------ pair kernels */
+
+/* separating_axis.py:58-75 polygons_intersect -> out (n) of 0/1 bytes. */
+int ct_polygons_intersect(const double* vertices_a, const double* vertices_b,
+                          const int32_t* faces_a, int32_t n_max_a, const int32_t* faces_b,
+                          int32_t n_max_b, const int64_t* indices_a, const int64_t* indices_b,
+                          int64_t n, uint8_t* out, void* stream);
+
+/* sutherland_hodgman.py:151-168 area_of_intersection (subject a, clipper b) -> out (n). */
+int ct_area_of_intersection(const double* vertices_a, const double* vertices_b,
+                            const int32_t* faces_a, int32_t n_max_a, const int32_t* faces_b,
+                            int32_t n_max_b, const int64_t* indices_a, const int64_t* indices_b,
+                            int64_t n, double* out, void* stream);
+
+/* sutherland_hodgman.py:171-187 box_area_of_intersection -> out (n). */
+int ct_box_area_of_intersection(const double* bbox_coords, const double* vertices,
+                                const int32_t* faces, int32_t n_max_vert,
+                                const int64_t* indices_bbox, const int64_t* indices_face,
+                                int64_t n, double* out, void* stream);
+
+/* celltree.py:186-269 locate_faces + intersect_faces, fused pair stage: for every bbox
+ * candidate pair k, keep[k] = SAT(a, b) (separating_axis.py:58-75) and, for intersect
+ * (with_area != 0), also area[k] = SH clip area and keep[k] &= area[k] > 0
+ * (celltree.py:266-269).  keep is int64 0/1 (it feeds ct_exclusive_scan). */
+int ct_face_pairs(const double* vertices_a, const int32_t* faces_a, int32_t n_max_a,
+                  const double* vertices_b, const int32_t* faces_b, int32_t n_max_b,
+                  const int64_t* ii, const int64_t* jj, int64_t n, int32_t with_area,
+                  double* area, int64_t* keep, void* stream);
+
+/* celltree.py:150-184 intersect_boxes pair stage: area[k] = box clip area, keep[k] = area > 0. */
+int ct_box_pairs(const double* bbox_coords, const double* vertices, const int32_t* faces,
+                 int32_t n_max_vert, const int64_t* ii, const int64_t* jj, int64_t n,
+                 double* area, int64_t* keep, void* stream);
+
+/* Stable compaction of (ii, jj, area): rows with keep[k] go to position offsets[k]
+ * (offsets from ct_exclusive_scan over keep).  area/area_out may be NULL. */
+int ct_compact_pairs(const int64_t* ii, const int64_t* jj, const double* area,
+                     const int64_t* keep, const int64_t* offsets, int64_t n, int64_t* ii_out,
+                     int64_t* jj_out, double* area_out, void* stream);
+
+/* ----------------------------------------------------------------- barycentric */
+
+/* barycentric_triangle.py:46-64 -> out (n, 3). */
+int ct_barycentric_triangle_weights(const double* points, const int64_t* face_indices,
+                                    int64_t n, const int32_t* faces, int32_t n_max_vert,
+                                    const double* vertices, double tol, double* out,
+                                    void* stream);
+
+/* barycentric_wachspress.py:88-107 -> out (n, n_max_vert). */
+int ct_barycentric_wachspress_weights(const double* points, const int64_t* face_indices,
+                                      int64_t n, const int32_t* faces, int32_t n_max_vert,
+                                      const double* vertices, double tol, double* out,
+                                      void* stream);
+
+/* celltree.py:306-351 compute_barycentric_weights, fused: locate each point (query.py:62-107)
+ * then its weights, Wachspress when n_max_vert > 3 else triangle.  face_out (n),
+ * weights (n, n_max_vert). */
+int ct_compute_barycentric_weights(const ct_tree* tree, const double* points, int64_t n,
+                                   double tol, int64_t* face_out, double* weights,
+                                   void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CELLTREE_HIP_H */

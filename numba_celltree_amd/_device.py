@@ -1,0 +1,119 @@
+"""Device plumbing: torch tensors hold every device array, the HIP library computes.
+
+Inputs may be host arrays (numpy / lists) or device tensors.  ``Io`` remembers which kind
+came in so results go back the same way: numpy in, numpy out (the reference's contract);
+device tensor in, device tensor out (no PCIe round trip).
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from numba_celltree_amd import _native
+from numba_celltree_amd.constants import NodeDType
+
+
+def device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError(
+            "numba_celltree_amd computes on a ROCm GPU only, and none is visible "
+            "(torch.cuda.is_available() is False)"
+        )
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream() -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t) -> ctypes.c_void_p:
+    return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+def is_device_tensor(x) -> bool:
+    return isinstance(x, torch.Tensor) and x.is_cuda
+
+
+class Io:
+    """Tracks whether a call received device tensors, and converts results back."""
+
+    def __init__(self, *inputs):
+        self.device_io = any(is_device_tensor(x) for x in inputs)
+
+    def out(self, t: torch.Tensor):
+        if self.device_io:
+            return t
+        return t.cpu().numpy()
+
+
+def to_dev(x, dtype: torch.dtype) -> torch.Tensor:
+    """Contiguous, 32-byte aligned device tensor of ``dtype`` (copies host data).
+
+    The kernels read points, boxes and segments with 16- and 32-byte vector loads; a view
+    that starts mid-row is copied to a fresh (aligned) allocation.
+    """
+    dev = device()
+    if isinstance(x, torch.Tensor):
+        t = x.to(device=dev, dtype=dtype).contiguous()
+    else:
+        t = torch.from_numpy(np.ascontiguousarray(x)).to(device=dev, dtype=dtype).contiguous()
+    if t.data_ptr() % 32:
+        t = t.clone()
+    return t
+
+
+def faces_to_dev(faces) -> torch.Tensor:
+    """int32 device copy of a (n, n_max_vert) index array."""
+    if isinstance(faces, torch.Tensor):
+        f = faces
+    else:
+        f = torch.from_numpy(np.ascontiguousarray(faces))
+    if f.numel() and (int(f.max()) >= 2**31 or int(f.min()) < -1):
+        raise ValueError("element indices must lie in [-1, 2**31)")
+    return f.to(device=device(), dtype=torch.int32).contiguous()
+
+
+def empty(shape, dtype) -> torch.Tensor:
+    return torch.empty(shape, dtype=dtype, device=device())
+
+
+def exclusive_scan(counts: torch.Tensor):
+    """Offsets (n + 1, int64, device) of ``counts`` and the total (host int, synchronizes)."""
+    n = counts.numel()
+    offsets = empty(n + 1, torch.int64)
+    ws = empty(int(_native.lib().ct_scan_workspace_bytes(n)), torch.uint8)
+    _native.call("ct_exclusive_scan", ptr(counts), n, ptr(offsets), ptr(ws), ws.numel(), stream())
+    total = int(offsets[n].item())
+    return offsets, total
+
+
+# ------------------------------------------------------------------ node records
+def nodes_to_host(raw: torch.Tensor, n_nodes: int) -> np.ndarray:
+    """Device ``ct_node`` bytes -> the reference's packed NodeDType array."""
+    buf = raw[: 32 * n_nodes].cpu().numpy()
+    f = buf.view(np.float64).reshape(n_nodes, 4)
+    i = buf.view(np.int32).reshape(n_nodes, 8)
+    out = np.empty(n_nodes, dtype=NodeDType)
+    out["child"] = i[:, 4]
+    out["Lmax"] = f[:, 0]
+    out["Rmin"] = f[:, 1]
+    out["ptr"] = i[:, 5]
+    out["size"] = i[:, 6]
+    out["dim"] = i[:, 7] != 0
+    return out
+
+
+def nodes_to_device(nodes: np.ndarray) -> torch.Tensor:
+    """NodeDType array -> device ``ct_node`` bytes."""
+    n = len(nodes)
+    buf = np.zeros(32 * n, dtype=np.uint8)
+    f = buf.view(np.float64).reshape(n, 4)
+    i = buf.view(np.int32).reshape(n, 8)
+    f[:, 0] = nodes["Lmax"]
+    f[:, 1] = nodes["Rmin"]
+    i[:, 4] = nodes["child"]
+    i[:, 5] = nodes["ptr"]
+    i[:, 6] = nodes["size"]
+    i[:, 7] = nodes["dim"].astype(np.int32)
+    return torch.from_numpy(buf).to(device())

@@ -1,0 +1,143 @@
+"""ctypes binding of libcelltree_hip.so (the C ABI declared in include/celltree_hip.h).
+
+The library is the only compute path of this package: there is no CPU fallback.  When the
+shared object is missing, or no ROCm GPU is visible, every entry point raises.
+"""
+
+import ctypes
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcelltree_hip.so")
+
+CT_OK = 0
+CT_ERR_ARG = -1
+CT_ERR_HIP = -2
+CT_ERR_DEGENERATE = -3
+CT_ERR_DEPTH = -4
+CT_ERR_WORKSPACE = -5
+CT_MAX_N_VERTEX = 32
+CT_MAX_DEPTH = 254
+
+
+class CtNode(ctypes.Structure):
+    """Device node record (32 bytes), ``ct_node`` in the C header."""
+
+    _fields_ = [
+        ("Lmax", ctypes.c_double),
+        ("Rmin", ctypes.c_double),
+        ("child", ctypes.c_int32),
+        ("ptr", ctypes.c_int32),
+        ("size", ctypes.c_int32),
+        ("dim", ctypes.c_int32),
+    ]
+
+
+class CtTree(ctypes.Structure):
+    """``ct_tree``: device pointers of a built tree plus its sizes and bbox."""
+
+    _fields_ = [
+        ("vertices", ctypes.c_void_p),
+        ("elements", ctypes.c_void_p),
+        ("bb_coords", ctypes.c_void_p),
+        ("nodes", ctypes.c_void_p),
+        ("bb_indices", ctypes.c_void_p),
+        ("n_vertices", ctypes.c_int64),
+        ("n_elements", ctypes.c_int64),
+        ("n_nodes", ctypes.c_int64),
+        ("n_max_vert", ctypes.c_int32),
+        ("depth", ctypes.c_int32),
+        ("bbox", ctypes.c_double * 4),
+    ]
+
+
+assert ctypes.sizeof(CtNode) == 32
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_I32 = ctypes.c_int32
+_F64 = ctypes.c_double
+_SZ = ctypes.c_size_t
+_TP = ctypes.POINTER(CtTree)
+
+# name -> (restype, argtypes); mirrors include/celltree_hip.h one to one.
+SIGNATURES = {
+    "ct_last_error": (ctypes.c_char_p, []),
+    "ct_version": (ctypes.c_int, []),
+    "ct_counter_clockwise": (ctypes.c_int, [_P, _P, _I64, _I32, _P]),
+    "ct_build_face_bboxes": (ctypes.c_int, [_P, _I64, _I32, _P, _P, _P]),
+    "ct_build_edge_bboxes": (ctypes.c_int, [_P, _I64, _P, _F64, _P, _P]),
+    "ct_bbox_stats": (ctypes.c_int, [_P, _I64, _P, _P, _P]),
+    "ct_vertex_extent": (ctypes.c_int, [_P, _I64, _P, _P]),
+    "ct_max_nodes": (_I64, [_I64]),
+    "ct_build_workspace_bytes": (_SZ, [_I64, _I32]),
+    "ct_build": (
+        ctypes.c_int,
+        [_P, _I64, _I32, _I32, _P, _SZ, _P, _P, ctypes.POINTER(_I64), ctypes.POINTER(_I32), _P],
+    ),
+    "ct_collect_node_bounds": (ctypes.c_int, [_P, _I64, ctypes.POINTER(_F64), _P, _P, _P]),
+    "ct_validate_node_bounds": (ctypes.c_int, [_TP, _P, _P, _P]),
+    "ct_locate_points": (ctypes.c_int, [_TP, _P, _I64, _F64, _P, _P]),
+    "ct_locate_points_on_edge": (ctypes.c_int, [_TP, _P, _I64, _F64, _P, _P]),
+    "ct_locate_boxes_count": (ctypes.c_int, [_TP, _P, _I64, _P, _P]),
+    "ct_locate_boxes_write": (ctypes.c_int, [_TP, _P, _I64, _P, _P, _P, _P]),
+    "ct_locate_edges_count": (ctypes.c_int, [_TP, _P, _I64, _I32, _P, _P]),
+    "ct_locate_edges_write": (ctypes.c_int, [_TP, _P, _I64, _I32, _P, _P, _P, _P, _P]),
+    "ct_sort_intersections_by_edge": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P]),
+    "ct_scan_workspace_bytes": (_SZ, [_I64]),
+    "ct_exclusive_scan": (ctypes.c_int, [_P, _I64, _P, _P, _SZ, _P]),
+    "ct_polygons_intersect": (ctypes.c_int, [_P, _P, _P, _I32, _P, _I32, _P, _P, _I64, _P, _P]),
+    "ct_area_of_intersection": (ctypes.c_int, [_P, _P, _P, _I32, _P, _I32, _P, _P, _I64, _P, _P]),
+    "ct_box_area_of_intersection": (ctypes.c_int, [_P, _P, _P, _I32, _P, _P, _I64, _P, _P]),
+    "ct_face_pairs": (ctypes.c_int, [_P, _P, _I32, _P, _P, _I32, _P, _P, _I64, _I32, _P, _P, _P]),
+    "ct_box_pairs": (ctypes.c_int, [_P, _P, _P, _I32, _P, _P, _I64, _P, _P, _P]),
+    "ct_compact_pairs": (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _P, _P, _P, _P]),
+    "ct_barycentric_triangle_weights": (ctypes.c_int, [_P, _P, _I64, _P, _I32, _P, _F64, _P, _P]),
+    "ct_barycentric_wachspress_weights": (ctypes.c_int, [_P, _P, _I64, _P, _I32, _P, _F64, _P, _P]),
+    "ct_compute_barycentric_weights": (ctypes.c_int, [_TP, _P, _I64, _F64, _P, _P, _P]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+def lib() -> ctypes.CDLL:
+    """Load the shared library (no GPU work happens at load time)."""
+    global _lib
+    if _lib is None:
+        with _lock:
+            if _lib is None:
+                if not os.path.exists(LIB_PATH):
+                    raise ImportError(
+                        f"{LIB_PATH} is not built; build it with "
+                        "`python numba_celltree_amd/csrc/build.py`"
+                    )
+                L = ctypes.CDLL(LIB_PATH)
+                for name, (res, args) in SIGNATURES.items():
+                    fn = getattr(L, name)
+                    fn.restype = res
+                    fn.argtypes = args
+                _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    msg = lib().ct_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(rc: int, name: str) -> None:
+    """Raise the Python exception matching a CT_ERR_* status."""
+    if rc == CT_OK:
+        return
+    msg = f"{name}: {last_error()}"
+    if rc == CT_ERR_ARG:
+        raise ValueError(msg)
+    if rc == CT_ERR_DEGENERATE:
+        raise IndexError(msg)
+    raise RuntimeError(msg)
+
+
+def call(name: str, *args) -> None:
+    check(getattr(lib(), name)(*args), name)

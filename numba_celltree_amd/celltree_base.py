@@ -1,0 +1,191 @@
+"""Shared base of CellTree2d / EdgeCellTree2d (numba_celltree/celltree_base.py:20-114).
+
+A tree lives on the GPU: vertices, elements, bounding boxes, nodes and bb_indices are
+device tensors, built by the HIP library.  The reference's public attributes (``nodes``,
+``bb_indices``, ``bb_coords``, ``bbox``, ``bb_distances``, ``vertices``,
+``celltree_data``) are host numpy copies made on first access and cached, so an edit of
+``tree.nodes`` (as the reference's tests do) is seen by ``node_bounds`` and
+``validate_node_bounds``.  Queries always use the device tree as it was built.
+"""
+
+import ctypes
+from typing import Optional
+
+import numpy as np
+import torch
+
+from numba_celltree_amd import _device as D
+from numba_celltree_amd import _native
+from numba_celltree_amd.constants import MIN_TOLERANCE, TOLERANCE_FACTOR, CellTreeData
+
+
+def default_tolerance(bb_diagonal) -> float:  # celltree_base.py:51-52
+    return max(MIN_TOLERANCE, TOLERANCE_FACTOR * max(bb_diagonal))
+
+
+class CellTree2dBase:
+    # ------------------------------------------------------------------ build
+    def _build_tree(self, vertices: torch.Tensor, elements: torch.Tensor, bb_coords: torch.Tensor,
+                    n_buckets: int, cells_per_leaf: int) -> None:
+        n = int(elements.shape[0])
+        if n == 0:
+            raise ValueError("a cell tree needs at least one element")
+        L = _native.lib()
+        st = D.stream()
+        ws = D.empty(int(L.ct_build_workspace_bytes(n, n_buckets)), torch.uint8)
+        cap = int(L.ct_max_nodes(n))
+        nodes_raw = D.empty(32 * cap, torch.uint8)
+        bb_indices = D.empty(n, torch.int32)
+        n_nodes = ctypes.c_int64(0)
+        depth = ctypes.c_int32(0)
+        _native.call(
+            "ct_build", D.ptr(bb_coords), n, n_buckets, cells_per_leaf, D.ptr(ws), ws.numel(),
+            D.ptr(nodes_raw), D.ptr(bb_indices), ctypes.byref(n_nodes), ctypes.byref(depth), st,
+        )
+        del ws
+        distances = D.empty((n, 3), torch.float64)
+        stats = D.empty(5, torch.float64)
+        _native.call("ct_bbox_stats", D.ptr(bb_coords), n, D.ptr(distances), D.ptr(stats), st)
+        host = stats.cpu().numpy()
+
+        self.n_buckets = n_buckets
+        self.cells_per_leaf = cells_per_leaf
+        self.d_vertices = vertices
+        self.d_elements = elements
+        self.d_bb_coords = bb_coords
+        self.d_nodes = nodes_raw[: 32 * n_nodes.value]
+        self.d_bb_indices = bb_indices
+        self.d_bb_distances = distances
+        self.n_nodes = int(n_nodes.value)
+        self.depth = int(depth.value)
+        self._bbox = host[:4].copy()
+        self._max_diagonal = float(host[4])
+        t = _native.CtTree()
+        t.vertices = vertices.data_ptr()
+        t.elements = elements.data_ptr()
+        t.bb_coords = bb_coords.data_ptr()
+        t.nodes = self.d_nodes.data_ptr()
+        t.bb_indices = bb_indices.data_ptr()
+        t.n_vertices = int(vertices.shape[0])
+        t.n_elements = n
+        t.n_nodes = self.n_nodes
+        t.n_max_vert = int(elements.shape[1])
+        t.depth = self.depth
+        for k in range(4):
+            t.bbox[k] = float(self._bbox[k])
+        self._ctree = t
+        self._host = {}
+
+    @property
+    def tree_ref(self):
+        return ctypes.byref(self._ctree)
+
+    def _default_tolerance(self) -> float:
+        return max(MIN_TOLERANCE, TOLERANCE_FACTOR * self._max_diagonal)
+
+    # ------------------------------------------------------- host attributes
+    def _cached(self, key, make):
+        if key not in self._host:
+            self._host[key] = make()
+        return self._host[key]
+
+    @property
+    def nodes(self) -> np.ndarray:
+        return self._cached("nodes", lambda: D.nodes_to_host(self.d_nodes, self.n_nodes))
+
+    @property
+    def bb_indices(self) -> np.ndarray:
+        return self._cached("bb_indices", lambda: self.d_bb_indices.cpu().numpy().astype(np.intp))
+
+    @property
+    def bb_coords(self) -> np.ndarray:
+        return self._cached("bb_coords", lambda: self.d_bb_coords.cpu().numpy())
+
+    @property
+    def bb_distances(self) -> np.ndarray:
+        return self._cached("bb_distances", lambda: self.d_bb_distances.cpu().numpy())
+
+    @property
+    def bbox(self) -> np.ndarray:
+        return self._bbox
+
+    @property
+    def vertices(self) -> np.ndarray:
+        return self._cached("vertices", lambda: self.d_vertices.cpu().numpy())
+
+    def _elements_host(self) -> np.ndarray:
+        return self._cached("elements", lambda: self.d_elements.cpu().numpy().astype(np.intp))
+
+    @property
+    def celltree_data(self) -> CellTreeData:
+        return CellTreeData(
+            self._elements_host(), self.vertices, self.nodes, self.bb_indices, self.bb_coords,
+            self.bbox, self.cells_per_leaf,
+        )
+
+    # ------------------------------------------------------------ validation
+    def _nodes_for_check(self) -> torch.Tensor:
+        if "nodes" in self._host:  # the host copy may have been edited
+            return D.nodes_to_device(self._host["nodes"])
+        return self.d_nodes
+
+    @property
+    def node_bounds(self) -> np.ndarray:
+        """Bounds (xmin, xmax, ymin, ymax) of every node (query.py:568-622)."""
+        return self._node_bounds(self._nodes_for_check()).cpu().numpy()
+
+    def _node_bounds(self, d_nodes: torch.Tensor) -> torch.Tensor:
+        n = d_nodes.numel() // 32
+        out = D.empty((n, 4), torch.float64)
+        parent = D.empty(n, torch.int32)
+        bbox = (ctypes.c_double * 4)(*[float(x) for x in self._bbox])
+        _native.call("ct_collect_node_bounds", D.ptr(d_nodes), n, bbox, D.ptr(parent), D.ptr(out), D.stream())
+        return out
+
+    def validate_node_bounds(self) -> np.ndarray:
+        """Whether every node's children / cells lie in its bounds (query.py:625-664)."""
+        d_nodes = self._nodes_for_check()
+        nb = self._node_bounds(d_nodes)
+        n = d_nodes.numel() // 32
+        t = _native.CtTree.from_buffer_copy(self._ctree)
+        t.nodes = d_nodes.data_ptr()
+        t.n_nodes = n
+        out = D.empty(n, torch.uint8)
+        _native.call("ct_validate_node_bounds", ctypes.byref(t), D.ptr(nb), D.ptr(out), D.stream())
+        return out.cpu().numpy().astype(bool)
+
+    def to_dict_of_lists(self):  # celltree_base.py:83-114
+        d = {}
+        for parent_index, left in enumerate(self.nodes["child"]):
+            d[parent_index] = [] if left == -1 else [int(left), int(left) + 1]
+        return d
+
+    # ---------------------------------------------------------------- points
+    def _locate(self, fn: str, points, tolerance: Optional[float]):
+        from numba_celltree_amd.cast import cast_vertices
+
+        if tolerance is None:
+            tolerance = self._default_tolerance()
+        io = D.Io(points)
+        pts = D.to_dev(cast_vertices(points), torch.float64)
+        n = int(pts.shape[0])
+        out = D.empty(n, torch.int64)
+        _native.call(fn, self.tree_ref, D.ptr(pts), n, float(tolerance), D.ptr(out), D.stream())
+        return io.out(out)
+
+    def _locate_edges(self, edge_coords, kind: int):
+        """count -> scan -> write, then the per-segment sort by t (device tensors)."""
+        n = int(edge_coords.shape[0])
+        st = D.stream()
+        counts = D.empty(n, torch.int64)
+        _native.call("ct_locate_edges_count", self.tree_ref, D.ptr(edge_coords), n, kind, D.ptr(counts), st)
+        offsets, total = D.exclusive_scan(counts)
+        ii = D.empty(total, torch.int64)
+        jj = D.empty(total, torch.int64)
+        xy = D.empty((total, 2, 2), torch.float64)
+        _native.call(
+            "ct_locate_edges_write", self.tree_ref, D.ptr(edge_coords), n, kind, D.ptr(offsets),
+            D.ptr(ii), D.ptr(jj), D.ptr(xy), st,
+        )
+        _native.call("ct_sort_intersections_by_edge", D.ptr(edge_coords), n, D.ptr(offsets), D.ptr(jj), D.ptr(xy), st)
+        return ii, jj, xy

@@ -1,0 +1,183 @@
+"""HIP library vs the CPU oracle on seeded synthetic meshes.
+
+Bar: integer and index results bit-exact, in the reference's order (the oracle
+reproduces the reference's chunk-concatenated order, so ordered arrays are compared, which
+implies the north star's sorted-set equality); FP64 results (areas, clipped points,
+weights) within 1e-12 relative, measured against max(|value|, scale).
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+from numba_celltree_amd import synthetic as S
+from oracle import refcpu as R
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-12
+
+
+def log(msg):
+    print(f"[gpu parity] {msg}", file=sys.stderr, flush=True)
+
+
+def assert_close(actual, expected, scale):
+    actual = np.asarray(actual)
+    expected = np.asarray(expected)
+    assert actual.shape == expected.shape
+    both_nan = np.isnan(actual) & np.isnan(expected)
+    err = np.where(both_nan, 0.0, np.abs(actual - expected))
+    bound = RTOL * np.maximum(np.abs(expected), scale)
+    assert np.all(err <= bound), f"max err {np.nanmax(err)} vs bound {bound.min()}"
+
+
+def same_tree(gpu, ora):
+    gn, on = gpu.nodes, ora.nodes
+    assert gn.shape == on.shape, (gn.shape, on.shape)
+    for field in ("child", "ptr", "size", "dim"):
+        assert np.array_equal(gn[field], on[field]), field
+    for field in ("Lmax", "Rmin"):
+        assert np.array_equal(gn[field].view(np.int64), on[field].view(np.int64)), field
+    assert np.array_equal(gpu.bb_indices, ora.bb_indices)
+    assert np.array_equal(gpu.bb_coords, ora.bb_coords)
+    assert np.array_equal(gpu.faces if hasattr(gpu, "faces") else gpu.edges,
+                          ora.faces if hasattr(ora, "faces") else ora.edges)
+
+
+MESHES = {
+    "cfg1_tri20k": lambda: S.config1()[:2],
+    "mixed_45k": lambda: S.config2_mesh(nv=151, seed=3),
+    "tri_voronoi": lambda: S.hex_voronoi(60, 50, 100.0, 0.1, 5),
+}
+
+
+@pytest.fixture(scope="module", params=list(MESHES))
+def mesh(request):
+    return request.param, MESHES[request.param]()
+
+
+@pytest.mark.parametrize("n_buckets,cells_per_leaf", [(4, 2), (2, 1), (8, 4), (3, 1)])
+def test_tree_structure_identical(mesh, n_buckets, cells_per_leaf):
+    import numba_celltree_amd as M
+
+    name, (v, f) = mesh
+    log(f"tree {name} nb={n_buckets} cpl={cells_per_leaf}")
+    g = M.CellTree2d(v, f, -1, n_buckets=n_buckets, cells_per_leaf=cells_per_leaf)
+    o = R.OracleCellTree2d(v, f, -1, n_buckets=n_buckets, cells_per_leaf=cells_per_leaf)
+    same_tree(g, o)
+    assert np.array_equal(g.node_bounds, o.node_bounds)
+    assert g.validate_node_bounds().all()
+    assert np.array_equal(g.bb_distances, o.bb_distances)
+    assert np.array_equal(g.bbox, o.bbox)
+
+
+@pytest.fixture(scope="module")
+def trees(mesh):
+    import numba_celltree_amd as M
+
+    name, (v, f) = mesh
+    return name, v, f, M.CellTree2d(v, f, -1), R.OracleCellTree2d(v, f, -1)
+
+
+def _extent(v):
+    return float(max(v[:, 0].max() - v[:, 0].min(), v[:, 1].max() - v[:, 1].min()))
+
+
+def test_locate_points(trees):
+    name, v, f, g, o = trees
+    L = _extent(v)
+    lo, hi = v.min(0), v.max(0)
+    rng = np.random.default_rng(1)
+    pts = rng.uniform(lo - 0.05 * L, hi + 0.05 * L, size=(200_000, 2))
+    # degenerate queries: every vertex, every edge midpoint, every cell centroid
+    fv = f[(f >= 0).all(1)] if f.shape[1] == 3 else f
+    a = v[fv[:, 0]]
+    b = v[fv[:, 1]]
+    special = np.concatenate([v, 0.5 * (a + b), v[f[:, :3]].mean(1)])
+    pts = np.concatenate([pts, special])
+    log(f"locate_points {name} n={len(pts)}")
+    for tol in (None, 1e-9, 1e-3 * L):
+        assert np.array_equal(g.locate_points(pts, tol), o.locate_points(pts, tol)), tol
+
+
+def test_locate_boxes_ordered(trees):
+    name, v, f, g, o = trees
+    L = _extent(v)
+    h = L / np.sqrt(len(f))
+    bx = S.boxes(20_000, L, h, 0.1, 10.0, 4) + np.array([v[:, 0].min()] * 2 + [v[:, 1].min()] * 2)
+    log(f"locate_boxes {name}")
+    gi, gj = g.locate_boxes(bx)
+    oi, oj = o.locate_boxes(bx)
+    assert np.array_equal(gi, oi) and np.array_equal(gj, oj)
+    gi, gj, ga = g.intersect_boxes(bx)
+    oi, oj, oa = o.intersect_boxes(bx)
+    assert np.array_equal(gi, oi) and np.array_equal(gj, oj)
+    assert_close(ga, oa, h * h)
+
+
+def test_intersect_edges_ordered(trees):
+    name, v, f, g, o = trees
+    L = _extent(v)
+    h = L / np.sqrt(len(f))
+    seg = S.segments(20_000, L, h, 0.1, 20.0, 5) + v.min(0)[None, None, :]
+    # segments along mesh edges and through vertices (collinear / touching branches)
+    fv = f[:, :3]
+    along = np.stack([v[fv[:500, 0]], v[fv[:500, 1]]], axis=1)
+    through = np.stack([v[fv[500:1000, 0]] - 0.5 * h, v[fv[500:1000, 0]] + 0.5 * h], axis=1)
+    seg = np.concatenate([seg, along, through, along[:, ::-1]])
+    log(f"intersect_edges {name} n={len(seg)}")
+    gi, gj, gxy = g.intersect_edges(seg)
+    oi, oj, oxy = o.intersect_edges(seg)
+    assert np.array_equal(gi, oi) and np.array_equal(gj, oj)
+    assert_close(gxy, oxy, L)
+
+
+def test_intersect_faces(trees):
+    name, v, f, g, o = trees
+    L = _extent(v)
+    tv, tf = S.hex_voronoi(40, 40, L, 0.1, 9)
+    tv = tv + v.min(0)
+    log(f"intersect_faces {name}")
+    gi, gj, ga = g.intersect_faces(tv, tf, -1)
+    oi, oj, oa = o.intersect_faces(tv, tf, -1)
+    assert np.array_equal(gi, oi) and np.array_equal(gj, oj)
+    assert_close(ga, oa, (L / 40) ** 2)
+    gi, gj = g.locate_faces(tv, tf.astype(np.int64).copy())
+    oi, oj = o.locate_faces(tv, tf.astype(np.int64).copy())
+    assert np.array_equal(gi, oi) and np.array_equal(gj, oj)
+
+
+def test_barycentric(trees):
+    name, v, f, g, o = trees
+    lo, hi = v.min(0), v.max(0)
+    pts = np.random.default_rng(2).uniform(lo, hi, size=(100_000, 2))
+    pts = np.concatenate([pts, v[:1000]])
+    log(f"barycentric {name}")
+    gf, gw = g.compute_barycentric_weights(pts)
+    of, ow = o.compute_barycentric_weights(pts)
+    assert np.array_equal(gf, of)
+    assert_close(gw, ow, 1.0)
+
+
+def test_edge_tree(trees):
+    import numba_celltree_amd as M
+
+    name, v, f, _, _ = trees
+    fv = f[:, :3]
+    edges = np.unique(np.sort(np.concatenate([fv[:, [0, 1]], fv[:, [1, 2]]]), axis=1), axis=0)[:20_000]
+    log(f"edge tree {name} n={len(edges)}")
+    g = M.EdgeCellTree2d(v, edges)
+    o = R.OracleEdgeCellTree2d(v, edges)
+    same_tree(g, o)
+    L = _extent(v)
+    mid = 0.5 * (v[edges[:, 0]] + v[edges[:, 1]])
+    pts = np.concatenate([mid, v[edges[:, 0]], np.random.default_rng(3).uniform(v.min(0), v.max(0), (50_000, 2))])
+    assert np.array_equal(g.locate_points(pts), o.locate_points(pts))
+    h = L / np.sqrt(len(f))
+    seg = S.segments(5_000, L, h, 0.1, 10.0, 6) + v.min(0)[None, None, :]
+    gi, gj, gxy = g.intersect_edges(seg)
+    oi, oj, oxy = o.intersect_edges(seg)
+    assert np.array_equal(gi, oi) and np.array_equal(gj, oj)
+    assert_close(gxy, oxy, L)
