@@ -58,6 +58,9 @@ typedef struct ct_tree {
   int32_t n_max_vert;
   int32_t depth;             /* longest root-to-leaf path, in edges */
   double bbox[4];            /* host values: tree bounding box */
+  const double* leaf_coords; /* optional (may be NULL): (n_elements, n_max_vert, 2), row i =
+                                the vertices of cell bb_indices[i], unused slots NaN; see
+                                ct_pack_leaf_coords */
 } ct_tree;
 
 const char* ct_last_error(void);
@@ -118,34 +121,62 @@ int ct_validate_node_bounds(const ct_tree* tree, const double* node_bounds, uint
 int ct_locate_points(const ct_tree* tree, const double* points, int64_t n, double tol,
                      int64_t* out, void* stream);
 
+/* Layout helper, no reference counterpart: fills `out` (n_elements * n_max_vert * 2 doubles)
+ * with the vertices of every cell in bb_indices order (unused slots NaN), so traversals read
+ * a leaf's cells contiguously instead of through bb_indices -> faces -> vertices.  Requires
+ * 3 <= n_max_vert <= 8 and non-NaN vertex coordinates. */
+int ct_pack_leaf_coords(const ct_tree* tree, double* out, void* stream);
+
+size_t ct_locate_workspace_bytes(int64_t n);
+
+/* query.py:110-117 locate_points, same results as ct_locate_points: the queries are first
+ * ordered with ct_query_order (kind 0), so the lanes of a wavefront walk nearly the same
+ * root-to-leaf path; every result is written back to its query's own slot. */
+int ct_locate_points_ordered(const ct_tree* tree, const double* points, int64_t n, double tol,
+                             void* workspace, size_t workspace_bytes, int64_t* out,
+                             void* stream);
+
 /* query.py:167-174 locate_points_on_edge (EdgeCellTree2d). */
 int ct_locate_points_on_edge(const ct_tree* tree, const double* points, int64_t n, double tol,
                              int64_t* out, void* stream);
 
+/* Scheduling helper, no reference counterpart: a processing order for n queries (kind 0
+ * points (n, 2), 1 boxes (n, 4), 2 segments (n, 2, 2)) from a counting sort on a 20-bit key:
+ * points by the Morton key of a 1024 x 1024 grid over `bbox` (HOST array of 4); boxes and
+ * segments by a work class (log2 of their bbox area relative to `bbox`) above the Morton key
+ * of their centre.  Passing the order to a count / write pass gives each wavefront
+ * neighbouring queries of similar cost; results are unchanged by it. */
+size_t ct_order_workspace_bytes(int64_t n);
+int ct_query_order(int32_t kind, const double* queries, int64_t n, const double* bbox,
+                   void* workspace, size_t workspace_bytes, int32_t* order, void* stream);
+
 /* query.py:278-289 locate_boxes, first pass: counts[i] = number of tree cells whose bbox
- * strictly overlaps box i. */
-int ct_locate_boxes_count(const ct_tree* tree, const double* boxes, int64_t n, int64_t* counts,
-                          void* stream);
+ * strictly overlaps box i.  `order` (from ct_query_order, or NULL) only schedules. */
+int ct_locate_boxes_count(const ct_tree* tree, const double* boxes, int64_t n,
+                          const int32_t* order, int64_t* counts, void* stream);
 
 /* query.py:278-289 locate_boxes, second pass: writes pairs at offsets[i] (from
  * ct_exclusive_scan), in box order then per-box DFS order (the reference's order). */
 int ct_locate_boxes_write(const ct_tree* tree, const double* boxes, int64_t n,
-                          const int64_t* offsets, int64_t* ii, int64_t* jj, void* stream);
+                          const int32_t* order, const int64_t* offsets, int64_t* ii,
+                          int64_t* jj, void* stream);
 
 /* query.py:500-544 locate_edge_faces (kind 1) / locate_edge_edges (kind 0), first pass. */
 int ct_locate_edges_count(const ct_tree* tree, const double* segments, int64_t n, int32_t kind,
-                          int64_t* counts, void* stream);
+                          const int32_t* order, int64_t* counts, void* stream);
 
 /* Second pass: ii, jj and xy (n_found, 2, 2) in segment order then DFS order. */
 int ct_locate_edges_write(const ct_tree* tree, const double* segments, int64_t n, int32_t kind,
-                          const int64_t* offsets, int64_t* ii, int64_t* jj, double* xy,
-                          void* stream);
+                          const int32_t* order, const int64_t* offsets, int64_t* ii,
+                          int64_t* jj, double* xy, void* stream);
 
-/* geometry_utils.py:564-574 sort_intersections_by_edge on grouped output: within each
- * segment's range [offsets[i], offsets[i+1]) stably sorts jj and xy by
- * t = (xy[:,0] - a) . (b - a), NaN last, in place. */
-int ct_sort_intersections_by_edge(const double* segments, int64_t n, const int64_t* offsets,
-                                  int64_t* jj, double* xy, void* stream);
+/* geometry_utils.py:564-574 sort_intersections_by_edge on grouped output (ii ascending, the
+ * rows of segment s in [offsets[s], offsets[s+1])): writes jj and xy (n_rows, 2, 2) stably
+ * sorted within each segment by t = (xy[:,0] - a) . (b - a), NaN last, into jj_out / xy_out
+ * (ii is unchanged by the sort). */
+int ct_sort_intersections_by_edge(const double* segments, const int64_t* offsets,
+                                  const int64_t* ii, const int64_t* jj, const double* xy,
+                                  int64_t n_rows, int64_t* jj_out, double* xy_out, void* stream);
 
 /* ------------------------------------------------------------- scan / compaction */
 

@@ -88,6 +88,16 @@ def exclusive_scan(counts: torch.Tensor):
     return offsets, total
 
 
+def query_order(kind: int, queries: torch.Tensor, bbox) -> torch.Tensor:
+    """Processing order (int32, device) of n queries: kind 0 points, 1 boxes, 2 segments."""
+    n = int(queries.shape[0])
+    order = empty(n, torch.int32)
+    ws = empty(int(_native.lib().ct_order_workspace_bytes(n)), torch.uint8)
+    bb = (ctypes.c_double * 4)(*[float(x) for x in bbox])
+    _native.call("ct_query_order", kind, ptr(queries), n, bb, ptr(ws), ws.numel(), ptr(order), stream())
+    return order
+
+
 # ------------------------------------------------------------------ node records
 def nodes_to_host(raw: torch.Tensor, n_nodes: int) -> np.ndarray:
     """Device ``ct_node`` bytes -> the reference's packed NodeDType array."""

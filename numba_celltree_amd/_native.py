@@ -49,6 +49,7 @@ class CtTree(ctypes.Structure):
         ("n_max_vert", ctypes.c_int32),
         ("depth", ctypes.c_int32),
         ("bbox", ctypes.c_double * 4),
+        ("leaf_coords", ctypes.c_void_p),
     ]
 
 
@@ -79,12 +80,17 @@ SIGNATURES = {
     "ct_collect_node_bounds": (ctypes.c_int, [_P, _I64, ctypes.POINTER(_F64), _P, _P, _P]),
     "ct_validate_node_bounds": (ctypes.c_int, [_TP, _P, _P, _P]),
     "ct_locate_points": (ctypes.c_int, [_TP, _P, _I64, _F64, _P, _P]),
+    "ct_pack_leaf_coords": (ctypes.c_int, [_TP, _P, _P]),
+    "ct_locate_workspace_bytes": (_SZ, [_I64]),
+    "ct_locate_points_ordered": (ctypes.c_int, [_TP, _P, _I64, _F64, _P, _SZ, _P, _P]),
     "ct_locate_points_on_edge": (ctypes.c_int, [_TP, _P, _I64, _F64, _P, _P]),
-    "ct_locate_boxes_count": (ctypes.c_int, [_TP, _P, _I64, _P, _P]),
-    "ct_locate_boxes_write": (ctypes.c_int, [_TP, _P, _I64, _P, _P, _P, _P]),
-    "ct_locate_edges_count": (ctypes.c_int, [_TP, _P, _I64, _I32, _P, _P]),
-    "ct_locate_edges_write": (ctypes.c_int, [_TP, _P, _I64, _I32, _P, _P, _P, _P, _P]),
-    "ct_sort_intersections_by_edge": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P]),
+    "ct_order_workspace_bytes": (_SZ, [_I64]),
+    "ct_query_order": (ctypes.c_int, [_I32, _P, _I64, ctypes.POINTER(_F64), _P, _SZ, _P, _P]),
+    "ct_locate_boxes_count": (ctypes.c_int, [_TP, _P, _I64, _P, _P, _P]),
+    "ct_locate_boxes_write": (ctypes.c_int, [_TP, _P, _I64, _P, _P, _P, _P, _P]),
+    "ct_locate_edges_count": (ctypes.c_int, [_TP, _P, _I64, _I32, _P, _P, _P]),
+    "ct_locate_edges_write": (ctypes.c_int, [_TP, _P, _I64, _I32, _P, _P, _P, _P, _P, _P]),
+    "ct_sort_intersections_by_edge": (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _P, _P, _P]),
     "ct_scan_workspace_bytes": (_SZ, [_I64]),
     "ct_exclusive_scan": (ctypes.c_int, [_P, _I64, _P, _P, _SZ, _P]),
     "ct_polygons_intersect": (ctypes.c_int, [_P, _P, _P, _I32, _P, _I32, _P, _P, _I64, _P, _P]),

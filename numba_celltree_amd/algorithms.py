@@ -1,10 +1,12 @@
 """Batch geometry kernels with the signatures of numba_celltree.algorithms
 (numba_celltree/algorithms/__init__.py:1-23) and numba_celltree.geometry_utils, on the GPU.
 
-Host inputs give host outputs; device tensors give device tensors.
+Host inputs give host outputs; device tensors give device tensors.  Every device tensor
+handed to the library is bound to a local name until the call has been enqueued: a
+temporary freed earlier could be handed by torch's caching allocator to the next
+temporary of the same call.
 """
 
-import numpy as np
 import torch
 
 from numba_celltree_amd import _device as D
@@ -19,16 +21,20 @@ def _f64(x) -> torch.Tensor:
     return D.to_dev(x, torch.float64)
 
 
+def _pair_inputs(vertices_a, vertices_b, faces_a, faces_b, indices_a, indices_b):
+    return (_f64(vertices_a), _f64(vertices_b), D.faces_to_dev(faces_a), D.faces_to_dev(faces_b),
+            _idx(indices_a), _idx(indices_b))
+
+
 def polygons_intersect(vertices_a, vertices_b, faces_a, faces_b, indices_a, indices_b):
     """SAT intersection test per pair (separating_axis.py:58-75)."""
     io = D.Io(vertices_a, vertices_b, faces_a, faces_b, indices_a, indices_b)
-    fa, fb = D.faces_to_dev(faces_a), D.faces_to_dev(faces_b)
-    ia, ib = _idx(indices_a), _idx(indices_b)
+    va, vb, fa, fb, ia, ib = _pair_inputs(vertices_a, vertices_b, faces_a, faces_b, indices_a, indices_b)
     n = int(ia.numel())
     out = D.empty(n, torch.uint8)
     _native.call(
-        "ct_polygons_intersect", D.ptr(_f64(vertices_a)), D.ptr(_f64(vertices_b)), D.ptr(fa),
-        int(fa.shape[1]), D.ptr(fb), int(fb.shape[1]), D.ptr(ia), D.ptr(ib), n, D.ptr(out), D.stream(),
+        "ct_polygons_intersect", D.ptr(va), D.ptr(vb), D.ptr(fa), int(fa.shape[1]), D.ptr(fb),
+        int(fb.shape[1]), D.ptr(ia), D.ptr(ib), n, D.ptr(out), D.stream(),
     )
     return io.out(out.bool())
 
@@ -36,13 +42,12 @@ def polygons_intersect(vertices_a, vertices_b, faces_a, faces_b, indices_a, indi
 def area_of_intersection(vertices_a, vertices_b, faces_a, faces_b, indices_a, indices_b):
     """Sutherland-Hodgman overlap area per pair (sutherland_hodgman.py:151-168)."""
     io = D.Io(vertices_a, vertices_b, faces_a, faces_b, indices_a, indices_b)
-    fa, fb = D.faces_to_dev(faces_a), D.faces_to_dev(faces_b)
-    ia, ib = _idx(indices_a), _idx(indices_b)
+    va, vb, fa, fb, ia, ib = _pair_inputs(vertices_a, vertices_b, faces_a, faces_b, indices_a, indices_b)
     n = int(ia.numel())
     out = D.empty(n, torch.float64)
     _native.call(
-        "ct_area_of_intersection", D.ptr(_f64(vertices_a)), D.ptr(_f64(vertices_b)), D.ptr(fa),
-        int(fa.shape[1]), D.ptr(fb), int(fb.shape[1]), D.ptr(ia), D.ptr(ib), n, D.ptr(out), D.stream(),
+        "ct_area_of_intersection", D.ptr(va), D.ptr(vb), D.ptr(fa), int(fa.shape[1]), D.ptr(fb),
+        int(fb.shape[1]), D.ptr(ia), D.ptr(ib), n, D.ptr(out), D.stream(),
     )
     return io.out(out)
 
@@ -50,44 +55,43 @@ def area_of_intersection(vertices_a, vertices_b, faces_a, faces_b, indices_a, in
 def box_area_of_intersection(bbox_coords, vertices, faces, indices_bbox, indices_face):
     """Overlap area of axis-aligned boxes with faces (sutherland_hodgman.py:171-187)."""
     io = D.Io(bbox_coords, vertices, faces, indices_bbox, indices_face)
-    f = D.faces_to_dev(faces)
+    bx, v, f = _f64(bbox_coords), _f64(vertices), D.faces_to_dev(faces)
     ib, iface = _idx(indices_bbox), _idx(indices_face)
     n = int(ib.numel())
     out = D.empty(n, torch.float64)
     _native.call(
-        "ct_box_area_of_intersection", D.ptr(_f64(bbox_coords)), D.ptr(_f64(vertices)), D.ptr(f),
-        int(f.shape[1]), D.ptr(ib), D.ptr(iface), n, D.ptr(out), D.stream(),
+        "ct_box_area_of_intersection", D.ptr(bx), D.ptr(v), D.ptr(f), int(f.shape[1]), D.ptr(ib),
+        D.ptr(iface), n, D.ptr(out), D.stream(),
     )
     return io.out(out)
 
 
-def _bary(fn, points, face_indices, faces, vertices, tolerance, ncols):
+def _bary(fn, points, face_indices, faces, vertices, tolerance, wachspress: bool):
     io = D.Io(points, face_indices, faces, vertices)
-    pts, fi, f = _f64(points), _idx(face_indices), D.faces_to_dev(faces)
+    pts, fi, f, v = _f64(points), _idx(face_indices), D.faces_to_dev(faces), _f64(vertices)
     n = int(pts.shape[0])
-    out = D.empty((n, ncols(int(f.shape[1]))), torch.float64)
-    _native.call(fn, D.ptr(pts), D.ptr(fi), n, D.ptr(f), int(f.shape[1]), D.ptr(_f64(vertices)),
-                 float(tolerance), D.ptr(out), D.stream())
+    nmax = int(f.shape[1])
+    out = D.empty((n, nmax if wachspress else 3), torch.float64)
+    _native.call(fn, D.ptr(pts), D.ptr(fi), n, D.ptr(f), nmax, D.ptr(v), float(tolerance), D.ptr(out), D.stream())
     return io.out(out)
 
 
 def barycentric_triangle_weights(points, face_indices, faces, vertices, tolerance):
     """(n, 3) triangle weights, zeros where face_indices == -1 (barycentric_triangle.py:46-64)."""
-    return _bary("ct_barycentric_triangle_weights", points, face_indices, faces, vertices, tolerance,
-                 lambda nmax: 3)
+    return _bary("ct_barycentric_triangle_weights", points, face_indices, faces, vertices, tolerance, False)
 
 
 def barycentric_wachspress_weights(points, face_indices, faces, vertices, tolerance):
     """(n, n_max_vert) Wachspress weights (barycentric_wachspress.py:88-107)."""
-    return _bary("ct_barycentric_wachspress_weights", points, face_indices, faces, vertices, tolerance,
-                 lambda nmax: nmax)
+    return _bary("ct_barycentric_wachspress_weights", points, face_indices, faces, vertices, tolerance, True)
 
 
 def counter_clockwise(vertices, faces) -> None:
     """Reorders clockwise face rows in place (geometry_utils.py:541-561)."""
+    v = _f64(vertices)
     d_faces = D.faces_to_dev(faces)
     n, nmax = int(d_faces.shape[0]), int(d_faces.shape[1])
-    _native.call("ct_counter_clockwise", D.ptr(_f64(vertices)), D.ptr(d_faces), n, nmax, D.stream())
+    _native.call("ct_counter_clockwise", D.ptr(v), D.ptr(d_faces), n, nmax, D.stream())
     if isinstance(faces, torch.Tensor):
         faces.copy_(d_faces.to(faces.dtype))
     else:
@@ -97,20 +101,20 @@ def counter_clockwise(vertices, faces) -> None:
 def build_face_bboxes(faces, vertices):
     """(n, 4) face bounding boxes (geometry_utils.py:443-456)."""
     io = D.Io(faces, vertices)
-    f = D.faces_to_dev(faces)
+    f, v = D.faces_to_dev(faces), _f64(vertices)
     n = int(f.shape[0])
     out = D.empty((n, 4), torch.float64)
-    _native.call("ct_build_face_bboxes", D.ptr(f), n, int(f.shape[1]), D.ptr(_f64(vertices)), D.ptr(out), D.stream())
+    _native.call("ct_build_face_bboxes", D.ptr(f), n, int(f.shape[1]), D.ptr(v), D.ptr(out), D.stream())
     return io.out(out)
 
 
 def build_edge_bboxes(edges, vertices, tolerance):
     """(n, 4) edge bounding boxes padded by tolerance (geometry_utils.py:473-487)."""
     io = D.Io(edges, vertices)
-    e = D.faces_to_dev(edges)
+    e, v = D.faces_to_dev(edges), _f64(vertices)
     n = int(e.shape[0])
     out = D.empty((n, 4), torch.float64)
-    _native.call("ct_build_edge_bboxes", D.ptr(e), n, D.ptr(_f64(vertices)), float(tolerance), D.ptr(out), D.stream())
+    _native.call("ct_build_edge_bboxes", D.ptr(e), n, D.ptr(v), float(tolerance), D.ptr(out), D.stream())
     return io.out(out)
 
 
@@ -118,12 +122,16 @@ def sort_intersections_by_edge(ii, jj, xy, edge_coords):
     """Stable sort by (edge, t) (geometry_utils.py:564-574); ``ii`` must be grouped by edge
     in ascending order, as every query of this package returns it."""
     io = D.Io(ii, jj, xy, edge_coords)
-    d_ii, d_jj, d_xy = _idx(ii), _idx(jj).clone(), _f64(xy).clone()
-    d_e = _f64(edge_coords)
+    d_ii, d_jj, d_xy, d_e = _idx(ii), _idx(jj), _f64(xy), _f64(edge_coords)
     n = int(d_e.shape[0])
     if d_ii.numel() > 1 and bool((d_ii[1:] < d_ii[:-1]).any()):
         raise ValueError("ii must be sorted ascending (grouped by edge)")
-    counts = torch.bincount(d_ii, minlength=n).to(torch.int64) if d_ii.numel() else torch.zeros(n, dtype=torch.int64, device=d_e.device)
+    counts = torch.bincount(d_ii, minlength=n).to(torch.int64)
     offsets, _ = D.exclusive_scan(counts)
-    _native.call("ct_sort_intersections_by_edge", D.ptr(d_e), n, D.ptr(offsets), D.ptr(d_jj), D.ptr(d_xy), D.stream())
-    return io.out(d_ii), io.out(d_jj), io.out(d_xy)
+    jj_out = torch.empty_like(d_jj)
+    xy_out = torch.empty_like(d_xy)
+    _native.call(
+        "ct_sort_intersections_by_edge", D.ptr(d_e), D.ptr(offsets), D.ptr(d_ii), D.ptr(d_jj), D.ptr(d_xy),
+        int(d_ii.numel()), D.ptr(jj_out), D.ptr(xy_out), D.stream(),
+    )
+    return io.out(d_ii), io.out(jj_out), io.out(xy_out)

@@ -57,19 +57,42 @@ class CellTree2d(CellTree2dBase):
         return self._elements_host()
 
     # ----------------------------------------------------------------- queries
+    # Point batches at least this large are processed in Morton order (ct_locate_points_ordered).
+    # Measured on MI355X at 1e7 uniform points / 1M cells the ordering costs more than it
+    # saves, so it is off by default (None).
+    ORDERED_MIN = None
+
     def locate_points(self, points, tolerance: Optional[float] = None):
         """Index of the face containing each point, -1 outside (celltree.py:99-128)."""
-        return self._locate("ct_locate_points", points, tolerance)
+        if tolerance is None:
+            tolerance = self._default_tolerance()
+        io = D.Io(points)
+        pts = D.to_dev(cast_vertices(points), torch.float64)
+        n = int(pts.shape[0])
+        out = D.empty(n, torch.int64)
+        if self.ORDERED_MIN is not None and n >= self.ORDERED_MIN:
+            ws = D.empty(int(_native.lib().ct_locate_workspace_bytes(n)), torch.uint8)
+            _native.call(
+                "ct_locate_points_ordered", self.tree_ref, D.ptr(pts), n, float(tolerance),
+                D.ptr(ws), ws.numel(), D.ptr(out), D.stream(),
+            )
+        else:
+            _native.call("ct_locate_points", self.tree_ref, D.ptr(pts), n, float(tolerance), D.ptr(out), D.stream())
+        return io.out(out)
 
     def _locate_boxes_dev(self, d_boxes: torch.Tensor):
         n = int(d_boxes.shape[0])
         st = D.stream()
+        order = D.query_order(1, d_boxes, self._bbox) if n >= self.ORDER_MIN else None
         counts = D.empty(n, torch.int64)
-        _native.call("ct_locate_boxes_count", self.tree_ref, D.ptr(d_boxes), n, D.ptr(counts), st)
+        _native.call("ct_locate_boxes_count", self.tree_ref, D.ptr(d_boxes), n, D.ptr(order), D.ptr(counts), st)
         offsets, total = D.exclusive_scan(counts)
         ii = D.empty(total, torch.int64)
         jj = D.empty(total, torch.int64)
-        _native.call("ct_locate_boxes_write", self.tree_ref, D.ptr(d_boxes), n, D.ptr(offsets), D.ptr(ii), D.ptr(jj), st)
+        _native.call(
+            "ct_locate_boxes_write", self.tree_ref, D.ptr(d_boxes), n, D.ptr(order), D.ptr(offsets), D.ptr(ii),
+            D.ptr(jj), st,
+        )
         return ii, jj
 
     def locate_boxes(self, bbox_coords) -> Tuple:

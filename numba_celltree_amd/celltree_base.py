@@ -24,6 +24,10 @@ def default_tolerance(bb_diagonal) -> float:  # celltree_base.py:51-52
 
 
 class CellTree2dBase:
+    # Box and segment batches at least this large are processed in the order of
+    # ct_query_order (similar work per wavefront); results do not depend on it.
+    ORDER_MIN = 4096
+
     # ------------------------------------------------------------------ build
     def _build_tree(self, vertices: torch.Tensor, elements: torch.Tensor, bb_coords: torch.Tensor,
                     n_buckets: int, cells_per_leaf: int) -> None:
@@ -73,8 +77,16 @@ class CellTree2dBase:
         t.depth = self.depth
         for k in range(4):
             t.bbox[k] = float(self._bbox[k])
+        t.leaf_coords = None
         self._ctree = t
         self._host = {}
+        nmax = int(elements.shape[1])
+        self.d_leaf_coords = None
+        if 3 <= nmax <= 8:
+            # Leaf-ordered vertex copy: a leaf's cells become one contiguous read.
+            self.d_leaf_coords = D.empty((n, nmax, 2), torch.float64)
+            _native.call("ct_pack_leaf_coords", self.tree_ref, D.ptr(self.d_leaf_coords), st)
+            t.leaf_coords = self.d_leaf_coords.data_ptr()
 
     @property
     def tree_ref(self):
@@ -177,15 +189,23 @@ class CellTree2dBase:
         """count -> scan -> write, then the per-segment sort by t (device tensors)."""
         n = int(edge_coords.shape[0])
         st = D.stream()
+        order = D.query_order(2, edge_coords, self._bbox) if n >= self.ORDER_MIN else None
         counts = D.empty(n, torch.int64)
-        _native.call("ct_locate_edges_count", self.tree_ref, D.ptr(edge_coords), n, kind, D.ptr(counts), st)
+        _native.call(
+            "ct_locate_edges_count", self.tree_ref, D.ptr(edge_coords), n, kind, D.ptr(order), D.ptr(counts), st
+        )
         offsets, total = D.exclusive_scan(counts)
         ii = D.empty(total, torch.int64)
         jj = D.empty(total, torch.int64)
         xy = D.empty((total, 2, 2), torch.float64)
         _native.call(
-            "ct_locate_edges_write", self.tree_ref, D.ptr(edge_coords), n, kind, D.ptr(offsets),
+            "ct_locate_edges_write", self.tree_ref, D.ptr(edge_coords), n, kind, D.ptr(order), D.ptr(offsets),
             D.ptr(ii), D.ptr(jj), D.ptr(xy), st,
         )
-        _native.call("ct_sort_intersections_by_edge", D.ptr(edge_coords), n, D.ptr(offsets), D.ptr(jj), D.ptr(xy), st)
-        return ii, jj, xy
+        jj_sorted = torch.empty_like(jj)
+        xy_sorted = torch.empty_like(xy)
+        _native.call(
+            "ct_sort_intersections_by_edge", D.ptr(edge_coords), D.ptr(offsets), D.ptr(ii), D.ptr(jj),
+            D.ptr(xy), total, D.ptr(jj_sorted), D.ptr(xy_sorted), st,
+        )
+        return ii, jj_sorted, xy_sorted

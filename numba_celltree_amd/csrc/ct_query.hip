@@ -1,12 +1,17 @@
 // ct_query.hip -- tree traversals on the device.
 //
-// One lane per query, depth-first with a private stack sized from the tree depth
-// (template STACK >= depth + 1, so it can never overflow: every pop pushes at most
-// two children, one level deeper).  Visit order, push order and leaf predicates are the
-// reference's (query.py:62-455), so first-hit answers and the order of variable-length
-// results match it exactly.  Variable-length queries run twice: a count pass, a device
-// exclusive scan (ct_exclusive_scan), and a write pass at the scanned offsets -- no
-// atomic append, output order = query order then DFS order.
+// One lane per query, depth first.  The reference pushes both viable children and pops the
+// near one at once (query.py:93-105, 236-242, 447-453); here the near child is entered
+// directly and only the far child is deferred on the stack, which visits nodes in exactly
+// the same order with half the stack traffic.  At most one deferred node per tree level is
+// ever pending, so a stack of STACK >= depth entries cannot overflow; STACK is chosen per
+// tree from {16, 24, 32, 48, 64, 128, 256}.  Leaf predicates are the reference's, so
+// first-hit answers and the order of variable-length results match it exactly.
+//
+// Variable-length queries run twice: a count pass, a device exclusive scan
+// (ct_exclusive_scan), and a write pass at the scanned offsets -- no atomic append; output
+// order = query order, then DFS order.  Queries may be processed in any lane order
+// (`order`, from ct_query_order): every query writes only its own count / rows.
 #include "ct_internal.h"
 #include "ct_primitives.h"
 
@@ -19,14 +24,28 @@ struct TreeView {
   const double* bb;
   const ct_node* nodes;
   const int32_t* bb_indices;
+  const double* leaf;  // packed leaf-ordered coordinates, or null
   int nmax;
   int n_elements;
   Box bbox;
 };
 
 TreeView view(const ct_tree* t) {
-  return TreeView{t->vertices, t->elements, t->bb_coords, t->nodes, t->bb_indices, t->n_max_vert,
-                  (int)t->n_elements, Box{t->bbox[0], t->bbox[1], t->bbox[2], t->bbox[3]}};
+  return TreeView{t->vertices, t->elements, t->bb_coords, t->nodes, t->bb_indices, t->leaf_coords,
+                  t->n_max_vert, (int)t->n_elements,
+                  Box{t->bbox[0], t->bbox[1], t->bbox[2], t->bbox[3]}};
+}
+
+// Calls f.template run<S>() with the smallest stack class S >= depth.
+template <typename F>
+int with_stack(int depth, F&& f) {
+  if (depth <= 16) return f.template run<16>();
+  if (depth <= 24) return f.template run<24>();
+  if (depth <= 32) return f.template run<32>();
+  if (depth <= 48) return f.template run<48>();
+  if (depth <= 64) return f.template run<64>();
+  if (depth <= 128) return f.template run<128>();
+  return f.template run<256>();
 }
 
 struct NodeLd {
@@ -46,28 +65,60 @@ __device__ __forceinline__ P2 ld_point(const double* __restrict__ pts, int64_t i
   return {d.x, d.y};
 }
 
+__device__ __forceinline__ int64_t query_index(const int32_t* __restrict__ order, int64_t i) {
+  return order ? (int64_t)order[i] : i;
+}
+
+// Row k of the packed leaf coordinates: the cell's vertices, unused slots NaN.  The length
+// rule is polygon_length's (geometry_utils.py:72-79): the first empty slot at index >= 3.
+template <int MAXV>
+__device__ __forceinline__ int load_leaf_poly(const double* __restrict__ leaf, int nmax, int k,
+                                              P2 (&poly)[MAXV]) {
+  const double2* row = reinterpret_cast<const double2*>(leaf) + (int64_t)k * nmax;
+#pragma unroll
+  for (int i = 0; i < MAXV; ++i) {
+    if (i < nmax) {
+      const double2 d = row[i];
+      poly[i] = {d.x, d.y};
+    } else {
+      poly[i] = {0.0, 0.0};
+    }
+  }
+  int len = nmax < MAXV ? nmax : MAXV;
+#pragma unroll
+  for (int i = MAXV - 1; i >= 3; --i)
+    if (i < nmax && isnan(poly[i].x)) len = i;
+  return len;
+}
+
 // ----------------------------------------------------------------- points
 // query.py:62-107 locate_point (faces) / :120-164 locate_point_on_edge (edges).
-template <int MAXV, int STACK, bool ON_EDGE>
+template <int MAXV, int STACK, bool ON_EDGE, bool PACKED>
 __device__ __forceinline__ int locate_one(const TreeView& t, P2 p, double tol) {
   int stack[STACK];
   int sp = 0;
-  stack[sp++] = 0;
-  while (sp > 0) {
-    const int ni = stack[--sp];
-    const NodeLd nd = ld_node(t.nodes, ni);
+  int node = 0;
+  while (true) {
+    const NodeLd nd = ld_node(t.nodes, node);
     if (nd.child == -1) {
       for (int k = nd.ptr; k < nd.ptr + nd.size; ++k) {
-        const int c = t.bb_indices[k];
         if constexpr (ON_EDGE) {
+          const int c = t.bb_indices[k];
           const int2 e = reinterpret_cast<const int2*>(t.elements)[c];
           if (point_on_edge(p, ld_vertex(t.vertices, e.x), ld_vertex(t.vertices, e.y), tol)) return c;
+        } else if constexpr (PACKED) {
+          P2 poly[MAXV];
+          const int len = load_leaf_poly<MAXV>(t.leaf, t.nmax, k, poly);
+          if (point_in_poly_or_edge<MAXV>(p, poly, len, tol)) return t.bb_indices[k];
         } else {
+          const int c = t.bb_indices[k];
           P2 poly[MAXV];
           const int len = load_poly<MAXV>(t.elements, t.nmax, c, t.vertices, poly);
           if (point_in_poly_or_edge<MAXV>(p, poly, len, tol)) return c;
         }
       }
+      if (sp == 0) return -1;
+      node = stack[--sp];
       continue;
     }
     const double v = nd.dim ? p.y : p.x;
@@ -75,58 +126,218 @@ __device__ __forceinline__ int locate_one(const TreeView& t, P2 p, double tol) {
     const bool right = v >= nd.Rmin;
     const int lc = nd.child, rc = lc + 1;
     if (left && right) {
-      // The side with the larger margin is popped first (query.py:93-101).
-      if ((nd.Lmax - v) < (v - nd.Rmin)) {
-        stack[sp++] = lc;
-        stack[sp++] = rc;
-      } else {
-        stack[sp++] = rc;
-        stack[sp++] = lc;
-      }
+      // Both viable: the side with the larger margin goes first (query.py:93-101).
+      const bool right_first = (nd.Lmax - v) < (v - nd.Rmin);
+      stack[sp++] = right_first ? lc : rc;
+      node = right_first ? rc : lc;
     } else if (left) {
-      stack[sp++] = lc;
+      node = lc;
     } else if (right) {
-      stack[sp++] = rc;
+      node = rc;
+    } else {
+      if (sp == 0) return -1;
+      node = stack[--sp];
     }
   }
-  return -1;
 }
 
-template <int MAXV, int STACK, bool ON_EDGE>
+// One lane per query; lane i answers query order[i] (or i).
+template <int MAXV, int STACK, bool ON_EDGE, bool PACKED>
 __global__ void __launch_bounds__(kBlock) k_locate_points(TreeView t, const double* __restrict__ pts,
+                                                          const int32_t* __restrict__ order,
                                                           int64_t n, double tol,
                                                           int64_t* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  out[i] = locate_one<MAXV, STACK, ON_EDGE>(t, ld_point(pts, i), tol);
+  const int64_t q = query_index(order, i);
+  out[q] = locate_one<MAXV, STACK, ON_EDGE, PACKED>(t, ld_point(pts, q), tol);
+}
+
+// ----------------------------------------------------------------- query ordering
+// Counting sort of the queries into 2^20 buckets.  Points: the Morton key of a 1024 x 1024
+// grid over the tree bbox.  Boxes and segments: a 4-bit work class (log2 of the query's
+// bbox area relative to the tree bbox; equal work per lane within a wavefront) above the
+// Morton key of a 256 x 256 grid of the query's centre.  Coordinates outside the tree bbox
+// clamp to the border.  The order within a bucket is whatever the atomics give: results
+// do not depend on the processing order.
+constexpr int kBuckets = 1 << 20;
+
+__device__ __forceinline__ unsigned spread_bits(unsigned v) {  // 10 bits -> even positions
+  v &= 0x3ffu;
+  v = (v | (v << 8)) & 0x00ff00ffu;
+  v = (v | (v << 4)) & 0x0f0f0f0fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  v = (v | (v << 1)) & 0x55555555u;
+  return v;
+}
+
+__device__ __forceinline__ unsigned grid_coord(double x, double lo, double scale, double cells) {
+  double g = (x - lo) * scale * cells;
+  g = g > 0.0 ? (g < cells - 1.0 ? g : cells - 1.0) : 0.0;  // NaN -> 0
+  return (unsigned)g;
+}
+
+struct OrderGeom {
+  Box bb;
+  double sx, sy;     // 1 / extent
+  double inv_area;   // 1 / (extent_x * extent_y)
+};
+
+__device__ __forceinline__ unsigned order_key(int kind, const double* __restrict__ q, int64_t i,
+                                              const OrderGeom& g) {
+  if (kind == 0) {
+    const double2 p = reinterpret_cast<const double2*>(q)[i];
+    return spread_bits(grid_coord(p.x, g.bb.xmin, g.sx, 1024.0)) |
+           (spread_bits(grid_coord(p.y, g.bb.ymin, g.sy, 1024.0)) << 1);
+  }
+  const double4 d = reinterpret_cast<const double4*>(q)[i];
+  double cx, cy, w, h;
+  if (kind == 1) {  // (xmin, xmax, ymin, ymax)
+    cx = 0.5 * (d.x + d.y);
+    cy = 0.5 * (d.z + d.w);
+    w = d.y - d.x;
+    h = d.w - d.z;
+  } else {  // ((x0, y0), (x1, y1))
+    cx = 0.5 * (d.x + d.z);
+    cy = 0.5 * (d.y + d.w);
+    w = fabs(d.z - d.x);
+    h = fabs(d.w - d.y);
+  }
+  const double rel = (w * h) * g.inv_area;  // fraction of the tree bbox covered
+  int cls;                                   // 2^-24 .. 2^-9 -> 0 .. 15
+  if (!(rel > 0.0)) {
+    cls = (rel == rel) ? 0 : 15;  // empty -> 0, NaN -> 15
+  } else if (rel >= 1.0) {
+    cls = 15;
+  } else {
+    cls = ilogb(rel) + 24;
+    cls = cls < 0 ? 0 : (cls > 15 ? 15 : cls);
+  }
+  const unsigned m = spread_bits(grid_coord(cx, g.bb.xmin, g.sx, 256.0)) |
+                     (spread_bits(grid_coord(cy, g.bb.ymin, g.sy, 256.0)) << 1);
+  return ((unsigned)cls << 16) | (m & 0xffffu);
+}
+
+__global__ void __launch_bounds__(kBlock) k_order_hist(int kind, const double* __restrict__ q, int64_t n,
+                                                       OrderGeom g, unsigned* __restrict__ hist) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  atomicAdd(hist + order_key(kind, q, i, g), 1u);
+}
+
+__global__ void __launch_bounds__(kBlock) k_order_widen(const unsigned* __restrict__ h, int64_t n,
+                                                        int64_t* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = h[i];
+}
+
+__global__ void __launch_bounds__(kBlock) k_order_scatter(int kind, const double* __restrict__ q, int64_t n,
+                                                          OrderGeom g,
+                                                          unsigned long long* __restrict__ cursor,
+                                                          int32_t* __restrict__ order) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned long long pos = atomicAdd(cursor + order_key(kind, q, i, g), 1ull);
+  order[pos] = (int32_t)i;
+}
+
+struct OrderWs {
+  unsigned* hist;
+  int64_t* wide;
+  unsigned long long* cursor;
+  void* scan_ws;
+  size_t scan_bytes;
+  int32_t* order;  // only used by ct_locate_points_ordered
+};
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+OrderWs carve_order(char* base, int64_t n_order, size_t* total) {
+  size_t off = 0;
+  OrderWs w{};
+  auto take = [&](size_t bytes) {
+    char* p = base ? base + off : nullptr;
+    off = align256(off + bytes);
+    return p;
+  };
+  w.hist = reinterpret_cast<unsigned*>(take(sizeof(unsigned) * kBuckets));
+  w.wide = reinterpret_cast<int64_t*>(take(sizeof(int64_t) * kBuckets));
+  w.cursor = reinterpret_cast<unsigned long long*>(take(sizeof(unsigned long long) * (kBuckets + 1)));
+  w.scan_bytes = ct_scan_workspace_bytes(kBuckets);
+  w.scan_ws = take(w.scan_bytes);
+  w.order = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * (size_t)(n_order > 0 ? n_order : 1)));
+  *total = off + 256;
+  return w;
+}
+
+int query_order(int kind, const double* q, int64_t n, const double* bbox, const OrderWs& w,
+                int32_t* order, hipStream_t st) {
+  OrderGeom g;
+  g.bb = Box{bbox[0], bbox[1], bbox[2], bbox[3]};
+  const double ex = g.bb.xmax - g.bb.xmin, ey = g.bb.ymax - g.bb.ymin;
+  g.sx = 1.0 / ex;
+  g.sy = 1.0 / ey;
+  g.inv_area = 1.0 / (ex * ey);
+  int rc = fill<unsigned>(w.hist, kBuckets, 0u, st);
+  if (rc) return rc;
+  k_order_hist<<<grid_for(n), kBlock, 0, st>>>(kind, q, n, g, w.hist);
+  CT_LAUNCH_CHECK();
+  k_order_widen<<<grid_for(kBuckets), kBlock, 0, st>>>(w.hist, kBuckets, w.wide);
+  CT_LAUNCH_CHECK();
+  rc = exclusive_scan(w.wide, kBuckets, reinterpret_cast<int64_t*>(w.cursor), w.scan_ws, w.scan_bytes, st);
+  if (rc) return rc;
+  k_order_scatter<<<grid_for(n), kBlock, 0, st>>>(kind, q, n, g, w.cursor, order);
+  CT_LAUNCH_CHECK();
+  return CT_OK;
+}
+
+// Packed leaf coordinates (see ct_pack_leaf_coords).
+__global__ void __launch_bounds__(kBlock) k_pack_leaf(TreeView t, double* __restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= t.n_elements) return;
+  const int c = t.bb_indices[k];
+  const int32_t* row = t.elements + (int64_t)c * t.nmax;
+  const int len = poly_len(row, t.nmax);
+  double2* dst = reinterpret_cast<double2*>(out) + k * t.nmax;
+  const double nan = __builtin_nan("");
+  for (int i = 0; i < t.nmax; ++i) {
+    if (i < len) {
+      const P2 v = ld_vertex(t.vertices, row[i]);
+      dst[i] = make_double2(v.x, v.y);
+    } else {
+      dst[i] = make_double2(nan, nan);
+    }
+  }
 }
 
 // ----------------------------------------------------------------- boxes
-// query.py:177-244 locate_box: strict overlap with the tree bbox first; left pushed before
-// right, so the right child is popped first.
+// query.py:177-244 locate_box: strict overlap with the tree bbox first; the left child is
+// pushed before the right, so the right child is visited first.
 template <bool WRITE, int STACK>
 __global__ void __launch_bounds__(kBlock) k_boxes(TreeView t, const double* __restrict__ boxes,
-                                                  int64_t n, int64_t* __restrict__ counts,
+                                                  const int32_t* __restrict__ order, int64_t n,
+                                                  int64_t* __restrict__ counts,
                                                   const int64_t* __restrict__ offsets,
                                                   int64_t* __restrict__ ii, int64_t* __restrict__ jj) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const Box box = ld_box(boxes, i);
+  const int64_t q = query_index(order, i);
+  const Box box = ld_box(boxes, q);
   int64_t cnt = 0;
-  int64_t o = WRITE ? offsets[i] : 0;
+  int64_t o = WRITE ? offsets[q] : 0;
   if (boxes_intersect(box, t.bbox)) {
     int stack[STACK];
     int sp = 0;
-    stack[sp++] = 0;
-    while (sp > 0) {
-      const int ni = stack[--sp];
-      const NodeLd nd = ld_node(t.nodes, ni);
+    int node = 0;
+    while (true) {
+      const NodeLd nd = ld_node(t.nodes, node);
+      bool pop = true;
       if (nd.child == -1) {
         for (int k = nd.ptr; k < nd.ptr + nd.size; ++k) {
           const int c = t.bb_indices[k];
           if (boxes_intersect(box, ld_box(t.bb, c))) {
             if (WRITE) {
-              ii[o] = i;
+              ii[o] = q;
               jj[o] = c;
               ++o;
             } else {
@@ -134,17 +345,27 @@ __global__ void __launch_bounds__(kBlock) k_boxes(TreeView t, const double* __re
             }
           }
         }
-        continue;
+      } else {
+        const double bmin = nd.dim ? box.ymin : box.xmin;
+        const double bmax = nd.dim ? box.ymax : box.xmax;
+        const bool left = bmin <= nd.Lmax;
+        const bool right = bmax >= nd.Rmin;
+        if (left && right) {
+          stack[sp++] = nd.child;
+          node = nd.child + 1;
+          pop = false;
+        } else if (left || right) {
+          node = nd.child + (right ? 1 : 0);
+          pop = false;
+        }
       }
-      const double bmin = nd.dim ? box.ymin : box.xmin;
-      const double bmax = nd.dim ? box.ymax : box.xmax;
-      const bool left = bmin <= nd.Lmax;
-      const bool right = bmax >= nd.Rmin;
-      if (left) stack[sp++] = nd.child;
-      if (right) stack[sp++] = nd.child + 1;
+      if (pop) {
+        if (sp == 0) break;
+        node = stack[--sp];
+      }
     }
   }
-  if (!WRITE) counts[i] = cnt;
+  if (!WRITE) counts[q] = cnt;
 }
 
 // ----------------------------------------------------------------- segments
@@ -174,25 +395,27 @@ __device__ __forceinline__ bool edge_edge(const TreeView& t, int c, P2 a, P2 b, 
 // per node (the t-tests of :428-441 always hold once the slab test passed, but are kept).
 template <bool WRITE, int MAXV, int STACK, bool FACES>
 __global__ void __launch_bounds__(kBlock) k_edges(TreeView t, const double* __restrict__ segs,
-                                                  int64_t n, int64_t* __restrict__ counts,
+                                                  const int32_t* __restrict__ order, int64_t n,
+                                                  int64_t* __restrict__ counts,
                                                   const int64_t* __restrict__ offsets,
                                                   int64_t* __restrict__ ii, int64_t* __restrict__ jj,
                                                   double* __restrict__ xy) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double4 s4 = reinterpret_cast<const double4*>(segs)[i];
+  const int64_t q = query_index(order, i);
+  const double4 s4 = reinterpret_cast<const double4*>(segs)[q];
   const P2 a = {s4.x, s4.y}, b = {s4.z, s4.w};
   int64_t cnt = 0;
-  int64_t o = WRITE ? offsets[i] : 0;
+  int64_t o = WRITE ? offsets[q] : 0;
   P2 c0, d0;
   if (cohen_sutherland(a, b, t.bbox, &c0, &d0)) {
     const double vx = b.x - a.x, vy = b.y - a.y;
     int stack[STACK];
     int sp = 0;
-    stack[sp++] = 0;
-    while (sp > 0) {
-      const int ni = stack[--sp];
-      const NodeLd nd = ld_node(t.nodes, ni);
+    int node = 0;
+    while (true) {
+      const NodeLd nd = ld_node(t.nodes, node);
+      bool pop = true;
       if (nd.child == -1) {
         for (int k = nd.ptr; k < nd.ptr + nd.size; ++k) {
           const int c = t.bb_indices[k];
@@ -200,7 +423,7 @@ __global__ void __launch_bounds__(kBlock) k_edges(TreeView t, const double* __re
           const bool hit = FACES ? edge_face<MAXV>(t, c, a, b, &cc, &dd) : edge_edge(t, c, a, b, &cc, &dd);
           if (hit) {
             if (WRITE) {
-              ii[o] = i;
+              ii[o] = q;
               jj[o] = c;
               reinterpret_cast<double4*>(xy)[o] = make_double4(cc.x, cc.y, dd.x, dd.y);
               ++o;
@@ -209,68 +432,78 @@ __global__ void __launch_bounds__(kBlock) k_edges(TreeView t, const double* __re
             }
           }
         }
-        continue;
-      }
-      const double dx = nd.dim ? vy : vx;
-      const double an = nd.dim ? a.y : a.x;
-      const double bn = nd.dim ? b.y : b.x;
-      double dl, dr;
-      if (dx > 0.0) {
-        dl = nd.Lmax - an;
-        dr = nd.Rmin - bn;
       } else {
-        dl = nd.Lmax - bn;
-        dr = nd.Rmin - an;
+        const double dx = nd.dim ? vy : vx;
+        const double an = nd.dim ? a.y : a.x;
+        const double bn = nd.dim ? b.y : b.x;
+        double dl, dr;
+        if (dx > 0.0) {
+          dl = nd.Lmax - an;
+          dr = nd.Rmin - bn;
+        } else {
+          dl = nd.Lmax - bn;
+          dr = nd.Rmin - an;
+        }
+        bool left = dl >= 0.0;
+        bool right = dr <= 0.0;
+        if (dx > 0.0) {
+          if (left) left = (dl / dx) >= 0.0;
+          if (right) right = (dr / dx) <= 1.0;
+        } else if (dx < 0.0) {
+          if (left) left = (1.0 - (dl / dx)) >= 0.0;
+          if (right) right = (1.0 - (dr / dx)) <= 1.0;
+        }
+        if (left && right) {
+          stack[sp++] = nd.child;
+          node = nd.child + 1;
+          pop = false;
+        } else if (left || right) {
+          node = nd.child + (right ? 1 : 0);
+          pop = false;
+        }
       }
-      bool left = dl >= 0.0;
-      bool right = dr <= 0.0;
-      if (dx > 0.0) {
-        if (left) left = (dl / dx) >= 0.0;
-        if (right) right = (dr / dx) <= 1.0;
-      } else if (dx < 0.0) {
-        if (left) left = (1.0 - (dl / dx)) >= 0.0;
-        if (right) right = (1.0 - (dr / dx)) <= 1.0;
+      if (pop) {
+        if (sp == 0) break;
+        node = stack[--sp];
       }
-      if (left) stack[sp++] = nd.child;
-      if (right) stack[sp++] = nd.child + 1;
     }
   }
-  if (!WRITE) counts[i] = cnt;
+  if (!WRITE) counts[q] = cnt;
 }
 
-// geometry_utils.py:564-574 sort_intersections_by_edge on grouped rows: a stable insertion
-// sort by t = (c - a) . (b - a) within each segment's row range; NaN sorts last (np.lexsort).
+// geometry_utils.py:564-574 sort_intersections_by_edge on grouped rows, np.lexsort's order
+// (stable, NaN last).  One lane per row: the row's rank inside its segment's range = rows
+// with a smaller t plus earlier rows with an equal t; the row is written once, to its rank.
+// t = (c - a) . (b - a) with c the clipped start point, x-term + y-term as the reference's
+// np.sum over the last axis.
 __device__ __forceinline__ bool t_less(double x, double y) {
   return x < y || (!isnan(x) && isnan(y));
 }
 
-__global__ void __launch_bounds__(kBlock) k_sort_by_edge(const double* __restrict__ segs, int64_t n,
+__global__ void __launch_bounds__(kBlock) k_sort_by_edge(const double* __restrict__ segs,
                                                          const int64_t* __restrict__ offsets,
-                                                         int64_t* __restrict__ jj,
-                                                         double* __restrict__ xy) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int64_t lo = offsets[i], hi = offsets[i + 1];
-  if (hi - lo < 2) return;
-  const double4 s4 = reinterpret_cast<const double4*>(segs)[i];
+                                                         const int64_t* __restrict__ ii,
+                                                         const int64_t* __restrict__ jj_in,
+                                                         const double* __restrict__ xy_in,
+                                                         int64_t n_rows, int64_t* __restrict__ jj_out,
+                                                         double* __restrict__ xy_out) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_rows) return;
+  const int64_t s = ii[r];
+  const int64_t lo = offsets[s], hi = offsets[s + 1];
+  const double4 s4 = reinterpret_cast<const double4*>(segs)[s];
   const double abx = s4.z - s4.x, aby = s4.w - s4.y;
-  double4* rows = reinterpret_cast<double4*>(xy);
-  auto tval = [&](const double4& r) { return (r.x - s4.x) * abx + (r.y - s4.y) * aby; };
-  for (int64_t k = lo + 1; k < hi; ++k) {
-    const double4 r = rows[k];
-    const int64_t j = jj[k];
-    const double t = tval(r);
-    int64_t m = k;
-    while (m > lo) {
-      const double4 q = rows[m - 1];
-      if (!t_less(t, tval(q))) break;
-      rows[m] = q;
-      jj[m] = jj[m - 1];
-      --m;
-    }
-    rows[m] = r;
-    jj[m] = j;
+  const double4 mine = reinterpret_cast<const double4*>(xy_in)[r];
+  const double t = (mine.x - s4.x) * abx + (mine.y - s4.y) * aby;
+  int64_t rank = lo;
+  for (int64_t k = lo; k < hi; ++k) {
+    if (k == r) continue;
+    const double2 c = reinterpret_cast<const double2*>(xy_in)[2 * k];
+    const double tk = (c.x - s4.x) * abx + (c.y - s4.y) * aby;
+    if (t_less(tk, t) || (k < r && !t_less(t, tk))) ++rank;
   }
+  jj_out[rank] = jj_in[r];
+  reinterpret_cast<double4*>(xy_out)[rank] = mine;
 }
 
 // ----------------------------------------------------------------- barycentric
@@ -290,7 +523,7 @@ __device__ __forceinline__ void tri_weights(P2 a, P2 b, P2 c, P2 p, double* w) {
   w[2] = ww;
 }
 
-// barycentric_wachspress.py:26-85 compute_weights (w has nw columns, zeroed by the caller).
+// barycentric_wachspress.py:26-85 compute_weights (w has nw columns).
 template <int MAXV>
 __device__ __forceinline__ void wachspress(const P2 (&poly)[MAXV], int n, P2 p, double* w, int nw,
                                            double tol) {
@@ -371,7 +604,7 @@ __global__ void __launch_bounds__(kBlock) k_bary(const double* __restrict__ pts,
   for (int k = 0; k < nw; ++k) out[i * nw + k] = w[k];
 }
 
-template <int MAXV, int STACK>
+template <int MAXV, int STACK, bool PACKED>
 __global__ void __launch_bounds__(kBlock) k_locate_bary(TreeView t, const double* __restrict__ pts,
                                                         int64_t n, double tol,
                                                         int64_t* __restrict__ face_out,
@@ -379,7 +612,7 @@ __global__ void __launch_bounds__(kBlock) k_locate_bary(TreeView t, const double
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const P2 p = ld_point(pts, i);
-  const int f = locate_one<MAXV, STACK, false>(t, p, tol);
+  const int f = locate_one<MAXV, STACK, false, PACKED>(t, p, tol);
   face_out[i] = f;
   const int nw = t.nmax;
   double w[MAXV];
@@ -457,18 +690,6 @@ __global__ void k_validate(TreeView t, int64_t n_nodes, const double* __restrict
 }
 
 // ----------------------------------------------------------------- dispatch
-template <int MAXV, bool ON_EDGE>
-int launch_locate_v(const TreeView& t, int depth, const double* pts, int64_t n, double tol,
-                    int64_t* out, hipStream_t st) {
-  const unsigned g = grid_for(n);
-  if (depth < 32) k_locate_points<MAXV, 32, ON_EDGE><<<g, kBlock, 0, st>>>(t, pts, n, tol, out);
-  else if (depth < 64) k_locate_points<MAXV, 64, ON_EDGE><<<g, kBlock, 0, st>>>(t, pts, n, tol, out);
-  else if (depth < 128) k_locate_points<MAXV, 128, ON_EDGE><<<g, kBlock, 0, st>>>(t, pts, n, tol, out);
-  else k_locate_points<MAXV, 256, ON_EDGE><<<g, kBlock, 0, st>>>(t, pts, n, tol, out);
-  CT_LAUNCH_CHECK();
-  return CT_OK;
-}
-
 int check_tree(const ct_tree* tree) {
   CT_REQUIRE(tree != nullptr && tree->nodes && tree->bb_indices && tree->elements && tree->vertices,
              "incomplete tree");
@@ -479,55 +700,118 @@ int check_tree(const ct_tree* tree) {
   return CT_OK;
 }
 
-template <bool WRITE>
-int launch_boxes(const ct_tree* tree, const double* boxes, int64_t n, int64_t* counts,
-                 const int64_t* offsets, int64_t* ii, int64_t* jj, hipStream_t st) {
-  const TreeView t = view(tree);
-  const unsigned g = grid_for(n);
-  const int d = tree->depth;
-  if (d < 32) k_boxes<WRITE, 32><<<g, kBlock, 0, st>>>(t, boxes, n, counts, offsets, ii, jj);
-  else if (d < 64) k_boxes<WRITE, 64><<<g, kBlock, 0, st>>>(t, boxes, n, counts, offsets, ii, jj);
-  else if (d < 128) k_boxes<WRITE, 128><<<g, kBlock, 0, st>>>(t, boxes, n, counts, offsets, ii, jj);
-  else k_boxes<WRITE, 256><<<g, kBlock, 0, st>>>(t, boxes, n, counts, offsets, ii, jj);
-  CT_LAUNCH_CHECK();
-  return CT_OK;
+template <int MAXV, bool ON_EDGE, bool PACKED>
+struct LocateLaunch {
+  const TreeView& t;
+  const double* pts;
+  const int32_t* order;
+  int64_t n;
+  double tol;
+  int64_t* out;
+  hipStream_t st;
+  template <int S>
+  int run() {
+    k_locate_points<MAXV, S, ON_EDGE, PACKED><<<grid_for(n), kBlock, 0, st>>>(t, pts, order, n, tol, out);
+    CT_LAUNCH_CHECK();
+    return CT_OK;
+  }
+};
+
+template <int MAXV, bool ON_EDGE, bool PACKED>
+int launch_locate_v(const TreeView& t, int depth, const double* pts, const int32_t* order, int64_t n,
+                    double tol, int64_t* out, hipStream_t st) {
+  return with_stack(depth, LocateLaunch<MAXV, ON_EDGE, PACKED>{t, pts, order, n, tol, out, st});
 }
+
+// Face trees: picks the vertex-count class, and the packed leaf path when present.
+int launch_locate_faces(const ct_tree* tree, const double* pts, const int32_t* order, int64_t n,
+                        double tol, int64_t* out, hipStream_t st) {
+  const TreeView t = view(tree);
+  const int nm = tree->n_max_vert, d = tree->depth;
+  if (t.leaf != nullptr && nm <= 8) {
+    if (nm <= 3) return launch_locate_v<3, false, true>(t, d, pts, order, n, tol, out, st);
+    if (nm <= 4) return launch_locate_v<4, false, true>(t, d, pts, order, n, tol, out, st);
+    return launch_locate_v<8, false, true>(t, d, pts, order, n, tol, out, st);
+  }
+  if (nm <= 3) return launch_locate_v<3, false, false>(t, d, pts, order, n, tol, out, st);
+  if (nm <= 4) return launch_locate_v<4, false, false>(t, d, pts, order, n, tol, out, st);
+  if (nm <= 8) return launch_locate_v<8, false, false>(t, d, pts, order, n, tol, out, st);
+  if (nm <= 16) return launch_locate_v<16, false, false>(t, d, pts, order, n, tol, out, st);
+  return launch_locate_v<32, false, false>(t, d, pts, order, n, tol, out, st);
+}
+
+template <bool WRITE>
+struct BoxesLaunch {
+  const TreeView& t;
+  const double* boxes;
+  const int32_t* order;
+  int64_t n;
+  int64_t* counts;
+  const int64_t* offsets;
+  int64_t* ii;
+  int64_t* jj;
+  hipStream_t st;
+  template <int S>
+  int run() {
+    k_boxes<WRITE, S><<<grid_for(n), kBlock, 0, st>>>(t, boxes, order, n, counts, offsets, ii, jj);
+    CT_LAUNCH_CHECK();
+    return CT_OK;
+  }
+};
 
 template <bool WRITE, int MAXV, bool FACES>
-int launch_edges_v(const TreeView& t, int d, const double* segs, int64_t n, int64_t* counts,
-                   const int64_t* offsets, int64_t* ii, int64_t* jj, double* xy, hipStream_t st) {
-  const unsigned g = grid_for(n);
-  if (d < 32) k_edges<WRITE, MAXV, 32, FACES><<<g, kBlock, 0, st>>>(t, segs, n, counts, offsets, ii, jj, xy);
-  else if (d < 64) k_edges<WRITE, MAXV, 64, FACES><<<g, kBlock, 0, st>>>(t, segs, n, counts, offsets, ii, jj, xy);
-  else if (d < 128) k_edges<WRITE, MAXV, 128, FACES><<<g, kBlock, 0, st>>>(t, segs, n, counts, offsets, ii, jj, xy);
-  else k_edges<WRITE, MAXV, 256, FACES><<<g, kBlock, 0, st>>>(t, segs, n, counts, offsets, ii, jj, xy);
-  CT_LAUNCH_CHECK();
-  return CT_OK;
-}
+struct EdgesLaunch {
+  const TreeView& t;
+  const double* segs;
+  const int32_t* order;
+  int64_t n;
+  int64_t* counts;
+  const int64_t* offsets;
+  int64_t* ii;
+  int64_t* jj;
+  double* xy;
+  hipStream_t st;
+  template <int S>
+  int run() {
+    k_edges<WRITE, MAXV, S, FACES><<<grid_for(n), kBlock, 0, st>>>(t, segs, order, n, counts, offsets, ii, jj, xy);
+    CT_LAUNCH_CHECK();
+    return CT_OK;
+  }
+};
 
 template <bool WRITE>
-int launch_edges(const ct_tree* tree, const double* segs, int64_t n, int kind, int64_t* counts,
-                 const int64_t* offsets, int64_t* ii, int64_t* jj, double* xy, hipStream_t st) {
+int launch_edges(const ct_tree* tree, const double* segs, const int32_t* order, int64_t n, int kind,
+                 int64_t* counts, const int64_t* offsets, int64_t* ii, int64_t* jj, double* xy,
+                 hipStream_t st) {
   const TreeView t = view(tree);
   const int d = tree->depth, nm = tree->n_max_vert;
-  if (kind == 0) return launch_edges_v<WRITE, 4, false>(t, d, segs, n, counts, offsets, ii, jj, xy, st);
-  if (nm <= 4) return launch_edges_v<WRITE, 4, true>(t, d, segs, n, counts, offsets, ii, jj, xy, st);
-  if (nm <= 8) return launch_edges_v<WRITE, 8, true>(t, d, segs, n, counts, offsets, ii, jj, xy, st);
-  if (nm <= 16) return launch_edges_v<WRITE, 16, true>(t, d, segs, n, counts, offsets, ii, jj, xy, st);
-  return launch_edges_v<WRITE, 32, true>(t, d, segs, n, counts, offsets, ii, jj, xy, st);
+  if (kind == 0)
+    return with_stack(d, EdgesLaunch<WRITE, 4, false>{t, segs, order, n, counts, offsets, ii, jj, xy, st});
+  if (nm <= 4)
+    return with_stack(d, EdgesLaunch<WRITE, 4, true>{t, segs, order, n, counts, offsets, ii, jj, xy, st});
+  if (nm <= 8)
+    return with_stack(d, EdgesLaunch<WRITE, 8, true>{t, segs, order, n, counts, offsets, ii, jj, xy, st});
+  if (nm <= 16)
+    return with_stack(d, EdgesLaunch<WRITE, 16, true>{t, segs, order, n, counts, offsets, ii, jj, xy, st});
+  return with_stack(d, EdgesLaunch<WRITE, 32, true>{t, segs, order, n, counts, offsets, ii, jj, xy, st});
 }
 
-template <int MAXV>
-int launch_locate_bary_v(const TreeView& t, int d, const double* pts, int64_t n, double tol,
-                         int64_t* fo, double* w, hipStream_t st) {
-  const unsigned g = grid_for(n);
-  if (d < 32) k_locate_bary<MAXV, 32><<<g, kBlock, 0, st>>>(t, pts, n, tol, fo, w);
-  else if (d < 64) k_locate_bary<MAXV, 64><<<g, kBlock, 0, st>>>(t, pts, n, tol, fo, w);
-  else if (d < 128) k_locate_bary<MAXV, 128><<<g, kBlock, 0, st>>>(t, pts, n, tol, fo, w);
-  else k_locate_bary<MAXV, 256><<<g, kBlock, 0, st>>>(t, pts, n, tol, fo, w);
-  CT_LAUNCH_CHECK();
-  return CT_OK;
-}
+template <int MAXV, bool PACKED>
+struct LocateBaryLaunch {
+  const TreeView& t;
+  const double* pts;
+  int64_t n;
+  double tol;
+  int64_t* fo;
+  double* w;
+  hipStream_t st;
+  template <int S>
+  int run() {
+    k_locate_bary<MAXV, S, PACKED><<<grid_for(n), kBlock, 0, st>>>(t, pts, n, tol, fo, w);
+    CT_LAUNCH_CHECK();
+    return CT_OK;
+  }
+};
 
 int launch_bary(const double* points, const int64_t* fi, int64_t n, const int32_t* faces, int nmax,
                 const double* vertices, double tol, int wachs, double* out, hipStream_t st) {
@@ -554,16 +838,63 @@ int ct_locate_points(const ct_tree* tree, const double* points, int64_t n, doubl
                      void* stream) {
   int rc = check_tree(tree);
   if (rc) return rc;
-  CT_REQUIRE(n >= 0, "n must be >= 0");
+  CT_REQUIRE(n >= 0 && tree->n_max_vert >= 3, "bad arguments");
   if (n == 0) return CT_OK;
-  const TreeView t = view(tree);
+  return launch_locate_faces(tree, points, nullptr, n, tol, out, as_stream(stream));
+}
+
+int ct_pack_leaf_coords(const ct_tree* tree, double* out, void* stream) {
+  int rc = check_tree(tree);
+  if (rc) return rc;
+  CT_REQUIRE(tree->n_max_vert >= 3 && tree->n_max_vert <= 8 && out, "n_max_vert must be in [3, 8]");
+  k_pack_leaf<<<grid_for(tree->n_elements), kBlock, 0, as_stream(stream)>>>(view(tree), out);
+  CT_LAUNCH_CHECK();
+  return CT_OK;
+}
+
+size_t ct_order_workspace_bytes(int64_t n) {
+  (void)n;
+  size_t total = 0;
+  carve_order(nullptr, 0, &total);
+  return total;
+}
+
+int ct_query_order(int32_t kind, const double* queries, int64_t n, const double* bbox, void* workspace,
+                   size_t workspace_bytes, int32_t* order, void* stream) {
+  CT_REQUIRE(kind >= 0 && kind <= 2 && n >= 0 && n < ((int64_t)1 << 31) && bbox && order,
+             "bad arguments");
+  if (n == 0) return CT_OK;
+  size_t need = 0;
+  OrderWs w = carve_order(static_cast<char*>(workspace), 0, &need);
+  if (workspace_bytes < need) {
+    set_error("order workspace too small: %zu < %zu", workspace_bytes, need);
+    return CT_ERR_WORKSPACE;
+  }
+  return query_order(kind, queries, n, bbox, w, order, as_stream(stream));
+}
+
+size_t ct_locate_workspace_bytes(int64_t n) {
+  size_t total = 0;
+  carve_order(nullptr, n < 0 ? 0 : n, &total);
+  return total;
+}
+
+int ct_locate_points_ordered(const ct_tree* tree, const double* points, int64_t n, double tol,
+                             void* workspace, size_t workspace_bytes, int64_t* out, void* stream) {
+  int rc = check_tree(tree);
+  if (rc) return rc;
+  CT_REQUIRE(n >= 0 && n < ((int64_t)1 << 31) && tree->n_max_vert >= 3, "bad arguments");
+  if (n == 0) return CT_OK;
+  size_t need = 0;
+  OrderWs w = carve_order(static_cast<char*>(workspace), n, &need);
+  if (workspace_bytes < need) {
+    set_error("locate workspace too small: %zu < %zu", workspace_bytes, need);
+    return CT_ERR_WORKSPACE;
+  }
   hipStream_t st = as_stream(stream);
-  const int nm = tree->n_max_vert, d = tree->depth;
-  if (nm <= 3) return launch_locate_v<3, false>(t, d, points, n, tol, out, st);
-  if (nm <= 4) return launch_locate_v<4, false>(t, d, points, n, tol, out, st);
-  if (nm <= 8) return launch_locate_v<8, false>(t, d, points, n, tol, out, st);
-  if (nm <= 16) return launch_locate_v<16, false>(t, d, points, n, tol, out, st);
-  return launch_locate_v<32, false>(t, d, points, n, tol, out, st);
+  rc = query_order(0, points, n, tree->bbox, w, w.order, st);
+  if (rc) return rc;
+  return launch_locate_faces(tree, points, w.order, n, tol, out, st);
 }
 
 int ct_locate_points_on_edge(const ct_tree* tree, const double* points, int64_t n, double tol,
@@ -573,54 +904,61 @@ int ct_locate_points_on_edge(const ct_tree* tree, const double* points, int64_t 
   CT_REQUIRE(n >= 0, "n must be >= 0");
   CT_REQUIRE(tree->n_max_vert == 2, "edge tree expected (n_max_vert == 2)");
   if (n == 0) return CT_OK;
-  return launch_locate_v<4, true>(view(tree), tree->depth, points, n, tol, out, as_stream(stream));
+  return launch_locate_v<4, true, false>(view(tree), tree->depth, points, nullptr, n, tol, out,
+                                         as_stream(stream));
 }
 
-int ct_locate_boxes_count(const ct_tree* tree, const double* boxes, int64_t n, int64_t* counts,
-                          void* stream) {
+int ct_locate_boxes_count(const ct_tree* tree, const double* boxes, int64_t n, const int32_t* order,
+                          int64_t* counts, void* stream) {
   int rc = check_tree(tree);
   if (rc) return rc;
   CT_REQUIRE(n >= 0, "n must be >= 0");
   if (n == 0) return CT_OK;
-  return launch_boxes<false>(tree, boxes, n, counts, nullptr, nullptr, nullptr, as_stream(stream));
+  const TreeView t = view(tree);
+  return with_stack(tree->depth, BoxesLaunch<false>{t, boxes, order, n, counts, nullptr, nullptr, nullptr,
+                                                    as_stream(stream)});
 }
 
-int ct_locate_boxes_write(const ct_tree* tree, const double* boxes, int64_t n, const int64_t* offsets,
-                          int64_t* ii, int64_t* jj, void* stream) {
+int ct_locate_boxes_write(const ct_tree* tree, const double* boxes, int64_t n, const int32_t* order,
+                          const int64_t* offsets, int64_t* ii, int64_t* jj, void* stream) {
   int rc = check_tree(tree);
   if (rc) return rc;
   CT_REQUIRE(n >= 0, "n must be >= 0");
   if (n == 0) return CT_OK;
-  return launch_boxes<true>(tree, boxes, n, nullptr, offsets, ii, jj, as_stream(stream));
+  const TreeView t = view(tree);
+  return with_stack(tree->depth, BoxesLaunch<true>{t, boxes, order, n, nullptr, offsets, ii, jj,
+                                                   as_stream(stream)});
 }
 
 int ct_locate_edges_count(const ct_tree* tree, const double* segments, int64_t n, int32_t kind,
-                          int64_t* counts, void* stream) {
+                          const int32_t* order, int64_t* counts, void* stream) {
   int rc = check_tree(tree);
   if (rc) return rc;
   CT_REQUIRE(n >= 0 && (kind == 0 || kind == 1), "bad arguments");
   CT_REQUIRE(kind == 1 || tree->n_max_vert == 2, "edge-edge queries need an edge tree");
   if (n == 0) return CT_OK;
-  return launch_edges<false>(tree, segments, n, kind, counts, nullptr, nullptr, nullptr, nullptr,
+  return launch_edges<false>(tree, segments, order, n, kind, counts, nullptr, nullptr, nullptr, nullptr,
                              as_stream(stream));
 }
 
 int ct_locate_edges_write(const ct_tree* tree, const double* segments, int64_t n, int32_t kind,
-                          const int64_t* offsets, int64_t* ii, int64_t* jj, double* xy,
-                          void* stream) {
+                          const int32_t* order, const int64_t* offsets, int64_t* ii, int64_t* jj,
+                          double* xy, void* stream) {
   int rc = check_tree(tree);
   if (rc) return rc;
   CT_REQUIRE(n >= 0 && (kind == 0 || kind == 1), "bad arguments");
   CT_REQUIRE(kind == 1 || tree->n_max_vert == 2, "edge-edge queries need an edge tree");
   if (n == 0) return CT_OK;
-  return launch_edges<true>(tree, segments, n, kind, nullptr, offsets, ii, jj, xy, as_stream(stream));
+  return launch_edges<true>(tree, segments, order, n, kind, nullptr, offsets, ii, jj, xy, as_stream(stream));
 }
 
-int ct_sort_intersections_by_edge(const double* segments, int64_t n, const int64_t* offsets,
-                                  int64_t* jj, double* xy, void* stream) {
-  CT_REQUIRE(n >= 0, "n must be >= 0");
-  if (n == 0) return CT_OK;
-  k_sort_by_edge<<<grid_for(n), kBlock, 0, as_stream(stream)>>>(segments, n, offsets, jj, xy);
+int ct_sort_intersections_by_edge(const double* segments, const int64_t* offsets, const int64_t* ii,
+                                  const int64_t* jj, const double* xy, int64_t n_rows,
+                                  int64_t* jj_out, double* xy_out, void* stream) {
+  CT_REQUIRE(n_rows >= 0, "n_rows must be >= 0");
+  if (n_rows == 0) return CT_OK;
+  k_sort_by_edge<<<grid_for(n_rows), kBlock, 0, as_stream(stream)>>>(segments, offsets, ii, jj, xy, n_rows,
+                                                                     jj_out, xy_out);
   CT_LAUNCH_CHECK();
   return CT_OK;
 }
@@ -649,11 +987,16 @@ int ct_compute_barycentric_weights(const ct_tree* tree, const double* points, in
   const TreeView t = view(tree);
   hipStream_t st = as_stream(stream);
   const int nm = tree->n_max_vert, d = tree->depth;
-  if (nm <= 3) return launch_locate_bary_v<3>(t, d, points, n, tol, face_out, weights, st);
-  if (nm <= 4) return launch_locate_bary_v<4>(t, d, points, n, tol, face_out, weights, st);
-  if (nm <= 8) return launch_locate_bary_v<8>(t, d, points, n, tol, face_out, weights, st);
-  if (nm <= 16) return launch_locate_bary_v<16>(t, d, points, n, tol, face_out, weights, st);
-  return launch_locate_bary_v<32>(t, d, points, n, tol, face_out, weights, st);
+  if (t.leaf != nullptr && nm <= 8) {
+    if (nm <= 3) return with_stack(d, LocateBaryLaunch<3, true>{t, points, n, tol, face_out, weights, st});
+    if (nm <= 4) return with_stack(d, LocateBaryLaunch<4, true>{t, points, n, tol, face_out, weights, st});
+    return with_stack(d, LocateBaryLaunch<8, true>{t, points, n, tol, face_out, weights, st});
+  }
+  if (nm <= 3) return with_stack(d, LocateBaryLaunch<3, false>{t, points, n, tol, face_out, weights, st});
+  if (nm <= 4) return with_stack(d, LocateBaryLaunch<4, false>{t, points, n, tol, face_out, weights, st});
+  if (nm <= 8) return with_stack(d, LocateBaryLaunch<8, false>{t, points, n, tol, face_out, weights, st});
+  if (nm <= 16) return with_stack(d, LocateBaryLaunch<16, false>{t, points, n, tol, face_out, weights, st});
+  return with_stack(d, LocateBaryLaunch<32, false>{t, points, n, tol, face_out, weights, st});
 }
 
 int ct_collect_node_bounds(const ct_node* nodes, int64_t n_nodes, const double* bbox,

@@ -102,6 +102,36 @@ def test_locate_points(trees):
         assert np.array_equal(g.locate_points(pts, tol), o.locate_points(pts, tol)), tol
 
 
+def test_locate_points_every_path(trees):
+    """Plain (unpacked leaves), plain (packed leaves) and ordered: identical answers."""
+    import torch
+
+    from numba_celltree_amd import _device as D
+    from numba_celltree_amd import _native
+
+    name, v, f, g, o = trees
+    pts = np.random.default_rng(7).uniform(v.min(0) - 1.0, v.max(0) + 1.0, size=(100_000, 2))
+    want = o.locate_points(pts)
+    tol = g._default_tolerance()
+    d_pts = torch.from_numpy(pts).cuda()
+    saved = g._ctree.leaf_coords
+    try:
+        for packed in (False, True):
+            g._ctree.leaf_coords = saved if packed else None
+            out = D.empty(len(pts), torch.int64)
+            _native.call("ct_locate_points", g.tree_ref, D.ptr(d_pts), len(pts), tol, D.ptr(out), D.stream())
+            assert np.array_equal(out.cpu().numpy(), want), f"plain packed={packed}"
+            ws = D.empty(int(_native.lib().ct_locate_workspace_bytes(len(pts))), torch.uint8)
+            out2 = D.empty(len(pts), torch.int64)
+            _native.call(
+                "ct_locate_points_ordered", g.tree_ref, D.ptr(d_pts), len(pts), tol, D.ptr(ws), ws.numel(),
+                D.ptr(out2), D.stream(),
+            )
+            assert np.array_equal(out2.cpu().numpy(), want), f"ordered packed={packed}"
+    finally:
+        g._ctree.leaf_coords = saved
+
+
 def test_locate_boxes_ordered(trees):
     name, v, f, g, o = trees
     L = _extent(v)
