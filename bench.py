@@ -109,7 +109,43 @@ def extras(args):
     out["locate_boxes_pairs"] = int(i.numel())
     i, _, _ = tree.intersect_edges(d_sg)
     out["intersect_edges_rows"] = int(i.numel())
-    del tree
+
+    # Regrid (16 fields) from the 999,941-cell mixed mesh to 250,000 hex-Voronoi cells:
+    # the plain forward against forward + hand-written adjoint (dF, d src vertices,
+    # d target vertices), topology fixed.
+    from numba_celltree_amd.regrid import Regridder
+
+    tv, tf = S.hex_voronoi(500, 500, 1000.0, 0.1, 7)
+    F = torch.from_numpy(S.sinusoid_fields(v[np.maximum(f, 0)].mean(1), 16, 1)).cuda()
+    d_tv = torch.from_numpy(tv).cuda()
+    d_tf = torch.from_numpy(tf).cuda()
+    t0 = time.perf_counter()
+    rg = Regridder(tree, d_tv, d_tf, -1)
+    torch.cuda.synchronize()
+    out["regrid_topology_ms"] = 1e3 * (time.perf_counter() - t0)
+    out["regrid_pairs"] = rg.topology.n_pairs
+    sv_t = tree.d_vertices.clone()
+    tv_t = rg.topology.tgt_vertices.clone()
+    G = torch.randn(rg.topology.n_tgt, 16, dtype=torch.float64, device="cuda")
+
+    def fwd():
+        with torch.no_grad():
+            return rg(F, sv_t, tv_t)
+
+    def fwd_bwd():
+        Fg = F.detach().requires_grad_(True)
+        svg = sv_t.detach().requires_grad_(True)
+        tvg = tv_t.detach().requires_grad_(True)
+        y = rg(Fg, svg, tvg)
+        (y * G).sum().backward()
+        return Fg.grad, svg.grad, tvg.grad
+
+    for name, fn in (("regrid_fwd_16f_ms", fwd), ("regrid_fwd_bwd_16f_ms", fwd_bwd)):
+        fn()
+        wall, _ = time_calls(fn, 5)
+        out[name] = 1e3 * wall / 5
+    out["regrid_fwd_bwd_over_fwd"] = out["regrid_fwd_bwd_16f_ms"] / out["regrid_fwd_16f_ms"]
+    del tree, rg
     sv, sf, tv, tf, _ = S.config3()
     t0 = time.perf_counter()
     src = M.CellTree2d(torch.from_numpy(sv).cuda(), torch.from_numpy(sf).cuda(), -1)
@@ -135,6 +171,10 @@ def main() -> int:
     ap.add_argument("--segments", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--only", choices=["all", "locate", "extras"], default="all",
+                    help="locate: the headline workload only; extras: the secondary workloads only")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run three 1 GiB device copies and exit (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -150,6 +190,19 @@ def main() -> int:
     from numba_celltree_amd import synthetic as S
 
     torch.cuda.set_device(0)
+    if args.calibrate_copy:
+        # A plain device copy of known size, for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE.
+        nbytes = 1 << 30
+        a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": nbytes, "copies": 3}), flush=True)
+        return 0
+    if args.only == "extras":
+        print(json.dumps({"extras": extras(args)}), flush=True)
+        return 0
     log("generating workload")
     v, f = S.config2_mesh()
     pts, _, _ = S.config2_queries(args.points, 10, 10)
@@ -163,14 +216,39 @@ def main() -> int:
     build_ms = 1e3 * (time.perf_counter() - t0)
     log(f"tree: {len(f)} cells, {tree.n_nodes} nodes, depth {tree.depth}, build {build_ms:.1f} ms")
 
-    fn = lambda: tree.locate_points(d_pts)  # noqa: E731
+    from numba_celltree_amd import _device as D
+
+    tol = tree._default_tolerance()
+    ordered = tree.ORDERED_MIN is not None and args.points >= tree.ORDERED_MIN
+
+    def step(ev=None):
+        # One locate_points call, split so the traversal kernel can be timed on its own:
+        # [ev0] spatial order (3 sort kernels + scan) [ev1] traversal kernel [ev2].
+        if ev:
+            ev[0].record()
+        order = D.query_order(0, d_pts, tree._bbox) if ordered else None
+        if ev:
+            ev[1].record()
+        out = tree._locate_points_dev(d_pts, tol, order)
+        if ev:
+            ev[2].record()
+        return out
+
     for _ in range(args.warmup):
-        fn()
+        step()
     torch.cuda.synchronize()
     log(f"timing {args.steps} steps")
-    wall, kernel_ms = time_calls(fn, args.steps)
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for ev in evs:
+        step(ev)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    order_ms = float(np.mean([a.elapsed_time(b) for a, b, _ in evs]))
+    kernel_ms = float(np.mean([b.elapsed_time(c) for _, b, c in evs]))
     ms_per_step = 1e3 * wall / args.steps
     value = args.steps * args.points / wall
+    fn = step
 
     n_cells, k = f.shape
     bytes_per_launch = args.points * 24 + 32 * tree.n_nodes + 4 * n_cells + 4 * k * n_cells + 16 * len(v)
@@ -182,7 +260,9 @@ def main() -> int:
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 5),
         "traffic": read_traffic("locate_points_cfg2_traffic.txt"),
+        "kernel": "k_locate_points (traversal)",
         "kernel_ms": round(kernel_ms, 4),
+        "order_ms": round(order_ms, 4),
         "algorithmic_bytes": int(bytes_per_launch),
     }
 
@@ -216,6 +296,8 @@ def main() -> int:
         "build_ms": round(build_ms, 2),
         "inside_fraction": float((res >= 0).mean()),
     }
+    if args.only == "locate":
+        args.no_cpu_baseline = args.no_extras = True
     if not args.no_cpu_baseline:
         log("cpu baseline")
         from oracle import refcpu as R

@@ -136,6 +136,11 @@ int ct_locate_points_ordered(const ct_tree* tree, const double* points, int64_t 
                              void* workspace, size_t workspace_bytes, int64_t* out,
                              void* stream);
 
+/* query.py:110-117 locate_points with a caller-supplied processing order (from
+ * ct_query_order kind 0, or any permutation of 0..n-1); same results as ct_locate_points. */
+int ct_locate_points_in_order(const ct_tree* tree, const double* points, int64_t n,
+                              const int32_t* order, double tol, int64_t* out, void* stream);
+
 /* query.py:167-174 locate_points_on_edge (EdgeCellTree2d). */
 int ct_locate_points_on_edge(const ct_tree* tree, const double* points, int64_t n, double tol,
                              int64_t* out, void* stream);
@@ -246,6 +251,43 @@ int ct_barycentric_wachspress_weights(const double* points, const int64_t* face_
 int ct_compute_barycentric_weights(const ct_tree* tree, const double* points, int64_t n,
                                    double tol, int64_t* face_out, double* weights,
                                    void* stream);
+
+/* ----------------------------------------------------- regridding (no reference counterpart)
+ * Conservative area-weighted regridding y[t,:] = sum_k A_k F[s_k,:] / area_t over a fixed pair
+ * list (tt[k] target, ss[k] source; grouped by target, ascending), the caller pattern of
+ * examples/spatial_indexing_2d_grids.py:268-323, and its adjoint under fixed topology.  A_k is
+ * the Sutherland-Hodgman area (subject = target face, clipper = source face, as in
+ * intersect_faces); faces are counter-clockwise rows of at most 8 vertices. */
+
+/* A_k for every pair. */
+int ct_regrid_pair_areas(const double* tgt_vertices, const int32_t* tgt_faces, int32_t n_max_t,
+                         const double* src_vertices, const int32_t* src_faces, int32_t n_max_s,
+                         const int64_t* tt, const int64_t* ss, int64_t n_pairs, double* area,
+                         void* stream);
+
+/* Fan area of every face (geometry_utils.py:82-95). */
+int ct_polygon_areas(const double* vertices, const int32_t* faces, int32_t n_max_vert, int64_t n,
+                     double* out, void* stream);
+
+/* y (n_tgt, n_fields) = segmented sum over each target's pairs [tgt_offsets[t], tgt_offsets[t+1]). */
+int ct_regrid_apply(const int64_t* tgt_offsets, const int64_t* ss, const double* area,
+                    const double* tgt_area, const double* F, int32_t n_fields, int64_t n_tgt,
+                    double* y, void* stream);
+
+/* gF (n_src, n_fields) = transpose, over perm (pairs sorted by source) and src_offsets. */
+int ct_regrid_apply_transpose(const int64_t* src_offsets, const int64_t* perm, const int64_t* tt,
+                              const double* area, const double* tgt_area, const double* gy,
+                              int32_t n_fields, int64_t n_src, double* gF, void* stream);
+
+/* Adds dL/d(tgt_vertices) and dL/d(src_vertices) for dL/dy = gy (caller zeroes the outputs):
+ * through every pair's clip (reverse sweep over its Sutherland-Hodgman stages) and through
+ * every target's area. */
+int ct_regrid_vertex_backward(const double* tgt_vertices, const int32_t* tgt_faces, int32_t n_max_t,
+                              const double* src_vertices, const int32_t* src_faces, int32_t n_max_s,
+                              const int64_t* tt, const int64_t* ss, int64_t n_pairs, int64_t n_tgt,
+                              const double* F, const double* y, const double* gy,
+                              const double* tgt_area, int32_t n_fields, double* g_tgt_vertices,
+                              double* g_src_vertices, void* stream);
 
 #ifdef __cplusplus
 }

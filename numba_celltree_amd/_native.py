@@ -83,6 +83,7 @@ SIGNATURES = {
     "ct_pack_leaf_coords": (ctypes.c_int, [_TP, _P, _P]),
     "ct_locate_workspace_bytes": (_SZ, [_I64]),
     "ct_locate_points_ordered": (ctypes.c_int, [_TP, _P, _I64, _F64, _P, _SZ, _P, _P]),
+    "ct_locate_points_in_order": (ctypes.c_int, [_TP, _P, _I64, _P, _F64, _P, _P]),
     "ct_locate_points_on_edge": (ctypes.c_int, [_TP, _P, _I64, _F64, _P, _P]),
     "ct_order_workspace_bytes": (_SZ, [_I64]),
     "ct_query_order": (ctypes.c_int, [_I32, _P, _I64, ctypes.POINTER(_F64), _P, _SZ, _P, _P]),
@@ -102,6 +103,14 @@ SIGNATURES = {
     "ct_barycentric_triangle_weights": (ctypes.c_int, [_P, _P, _I64, _P, _I32, _P, _F64, _P, _P]),
     "ct_barycentric_wachspress_weights": (ctypes.c_int, [_P, _P, _I64, _P, _I32, _P, _F64, _P, _P]),
     "ct_compute_barycentric_weights": (ctypes.c_int, [_TP, _P, _I64, _F64, _P, _P, _P]),
+    "ct_regrid_pair_areas": (ctypes.c_int, [_P, _P, _I32, _P, _P, _I32, _P, _P, _I64, _P, _P]),
+    "ct_polygon_areas": (ctypes.c_int, [_P, _P, _I32, _I64, _P, _P]),
+    "ct_regrid_apply": (ctypes.c_int, [_P, _P, _P, _P, _P, _I32, _I64, _P, _P]),
+    "ct_regrid_apply_transpose": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I32, _I64, _P, _P]),
+    "ct_regrid_vertex_backward": (
+        ctypes.c_int,
+        [_P, _P, _I32, _P, _P, _I32, _P, _P, _I64, _I64, _P, _P, _P, _P, _I32, _P, _P, _P],
+    ),
 }
 
 _lib = None

@@ -57,10 +57,9 @@ class CellTree2d(CellTree2dBase):
         return self._elements_host()
 
     # ----------------------------------------------------------------- queries
-    # Point batches at least this large are processed in Morton order (ct_locate_points_ordered).
-    # Measured on MI355X at 1e7 uniform points / 1M cells the ordering costs more than it
-    # saves, so it is off by default (None).
-    ORDERED_MIN = None
+    # Point batches at least this large are processed in Morton order (ct_query_order, then
+    # ct_locate_points_in_order): neighbouring lanes then share their root-to-leaf paths.
+    ORDERED_MIN = 1 << 18
 
     def locate_points(self, points, tolerance: Optional[float] = None):
         """Index of the face containing each point, -1 outside (celltree.py:99-128)."""
@@ -68,17 +67,20 @@ class CellTree2d(CellTree2dBase):
             tolerance = self._default_tolerance()
         io = D.Io(points)
         pts = D.to_dev(cast_vertices(points), torch.float64)
+        out = self._locate_points_dev(pts, float(tolerance))
+        return io.out(out)
+
+    def _locate_points_dev(self, pts: torch.Tensor, tolerance: float, order=None) -> torch.Tensor:
         n = int(pts.shape[0])
         out = D.empty(n, torch.int64)
-        if self.ORDERED_MIN is not None and n >= self.ORDERED_MIN:
-            ws = D.empty(int(_native.lib().ct_locate_workspace_bytes(n)), torch.uint8)
-            _native.call(
-                "ct_locate_points_ordered", self.tree_ref, D.ptr(pts), n, float(tolerance),
-                D.ptr(ws), ws.numel(), D.ptr(out), D.stream(),
-            )
+        if order is None and self.ORDERED_MIN is not None and n >= self.ORDERED_MIN:
+            order = D.query_order(0, pts, self._bbox)
+        if order is not None:
+            _native.call("ct_locate_points_in_order", self.tree_ref, D.ptr(pts), n, D.ptr(order), tolerance,
+                         D.ptr(out), D.stream())
         else:
-            _native.call("ct_locate_points", self.tree_ref, D.ptr(pts), n, float(tolerance), D.ptr(out), D.stream())
-        return io.out(out)
+            _native.call("ct_locate_points", self.tree_ref, D.ptr(pts), n, tolerance, D.ptr(out), D.stream())
+        return out
 
     def _locate_boxes_dev(self, d_boxes: torch.Tensor):
         n = int(d_boxes.shape[0])

@@ -9,9 +9,10 @@ namespace ct {
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
-// Exclusive scan over a 256-lane workgroup.  lds must hold kBlock/64 elements.  Every lane
-// must call it (it contains barriers).  *total receives the workgroup sum.
-template <typename T>
+// Exclusive scan over a BLOCK-lane workgroup (BLOCK a multiple of 64, at most 1024).  lds
+// must hold BLOCK/64 elements.  Every lane must call it (it contains barriers).  *total
+// receives the workgroup sum.
+template <typename T, int BLOCK = kBlock>
 __device__ __forceinline__ T block_exclusive_scan(T v, T* lds, T* total) {
   const int lane = lane_id(), w = threadIdx.x >> 6;
   T x = v;
@@ -24,7 +25,7 @@ __device__ __forceinline__ T block_exclusive_scan(T v, T* lds, T* total) {
   __syncthreads();
   T before = 0, tot = 0;
 #pragma unroll
-  for (int k = 0; k < kBlock / 64; ++k) {
+  for (int k = 0; k < BLOCK / 64; ++k) {
     const T s = lds[k];
     if (k < w) before += s;
     tot += s;

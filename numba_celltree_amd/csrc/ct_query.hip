@@ -141,27 +141,40 @@ __device__ __forceinline__ int locate_one(const TreeView& t, P2 p, double tol) {
   }
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on XCD b % 8).
+// With ordered queries, workgroup b takes tile (b % 8) * (grid / 8) + b / 8, so every XCD
+// sweeps one contiguous range of the order -- one region of the tree -- through its own L2.
+// The grid is a multiple of 8, which makes the map a bijection.
+constexpr unsigned kXcds = 8;
+
+__device__ __forceinline__ int64_t xcd_tile(unsigned b, unsigned grid) {
+  return (int64_t)(b % kXcds) * (grid / kXcds) + b / kXcds;
+}
+
+inline unsigned grid_xcd(int64_t n) {
+  const unsigned g = grid_for(n);
+  return (g + kXcds - 1) / kXcds * kXcds;
+}
+
 // One lane per query; lane i answers query order[i] (or i).
 template <int MAXV, int STACK, bool ON_EDGE, bool PACKED>
 __global__ void __launch_bounds__(kBlock) k_locate_points(TreeView t, const double* __restrict__ pts,
                                                           const int32_t* __restrict__ order,
                                                           int64_t n, double tol,
                                                           int64_t* __restrict__ out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int64_t q = query_index(order, i);
   out[q] = locate_one<MAXV, STACK, ON_EDGE, PACKED>(t, ld_point(pts, q), tol);
 }
 
 // ----------------------------------------------------------------- query ordering
-// Counting sort of the queries into 2^20 buckets.  Points: the Morton key of a 1024 x 1024
-// grid over the tree bbox.  Boxes and segments: a 4-bit work class (log2 of the query's
-// bbox area relative to the tree bbox; equal work per lane within a wavefront) above the
-// Morton key of a 256 x 256 grid of the query's centre.  Coordinates outside the tree bbox
-// clamp to the border.  The order within a bucket is whatever the atomics give: results
-// do not depend on the processing order.
-constexpr int kBuckets = 1 << 20;
-
+// Sort of the queries by a 20-bit key.  Points: the Morton key of a 1024 x 1024 grid over
+// the tree bbox.  Boxes and segments: a 4-bit work class (log2 of the query's bbox area
+// relative to the tree bbox; equal work per lane within a wavefront) above the Morton key
+// of a 256 x 256 grid of the query's centre.  Coordinates outside the tree bbox clamp to
+// the border.  The order among equal keys is whatever the LDS atomics give: results do not
+// depend on the processing order.
 __device__ __forceinline__ unsigned spread_bits(unsigned v) {  // 10 bits -> even positions
   v &= 0x3ffu;
   v = (v | (v << 8)) & 0x00ff00ffu;
@@ -218,41 +231,140 @@ __device__ __forceinline__ unsigned order_key(int kind, const double* __restrict
   return ((unsigned)cls << 16) | (m & 0xffffu);
 }
 
-__global__ void __launch_bounds__(kBlock) k_order_hist(int kind, const double* __restrict__ q, int64_t n,
-                                                       OrderGeom g, unsigned* __restrict__ hist) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  atomicAdd(hist + order_key(kind, q, i, g), 1u);
+// Two-level MSD bucket sort on the 20-bit key, without global atomics:
+//   k_sort_hist     per tile of kSortTile queries: LDS histogram of the top 8 key bits,
+//                   written bin-major to T[bin][tile]
+//   scan of T       -> the start of every (bin, tile) run
+//   k_sort_scatter  per tile: LDS cursors, (key, index) written to the coarse runs
+//   k_sort_fine     one workgroup per coarse bin: LDS histogram / scan / scatter of the
+//                   low 12 key bits -> the final order
+constexpr int kSortItems = 32;
+constexpr int kSortBatch = 8;  // loads in flight per lane
+constexpr int kSortTile = kBlock * kSortItems;
+constexpr int kCoarse = 256;  // == kBlock
+constexpr int kFineBits = 12;
+constexpr int kFine = 1 << kFineBits;
+constexpr int kFineBlock = 1024;
+
+__global__ void __launch_bounds__(kBlock) k_sort_hist(int kind, const double* __restrict__ q, int64_t n,
+                                                      OrderGeom g, int64_t ntiles,
+                                                      int64_t* __restrict__ T) {
+  __shared__ int hist[kCoarse];
+  hist[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kSortTile;
+  for (int j0 = 0; j0 < kSortItems; j0 += kSortBatch) {
+    unsigned k[kSortBatch];
+#pragma unroll
+    for (int j = 0; j < kSortBatch; ++j) {
+      const int64_t i = base + (int64_t)(j0 + j) * kBlock + threadIdx.x;
+      k[j] = (i < n) ? order_key(kind, q, i, g) : ~0u;
+    }
+#pragma unroll
+    for (int j = 0; j < kSortBatch; ++j)
+      if (k[j] != ~0u) atomicAdd(&hist[k[j] >> kFineBits], 1);
+  }
+  __syncthreads();
+  T[(int64_t)threadIdx.x * ntiles + blockIdx.x] = hist[threadIdx.x];
 }
 
-__global__ void __launch_bounds__(kBlock) k_order_widen(const unsigned* __restrict__ h, int64_t n,
-                                                        int64_t* __restrict__ out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = h[i];
+__global__ void __launch_bounds__(kBlock) k_sort_scatter(int kind, const double* __restrict__ q, int64_t n,
+                                                         OrderGeom g, int64_t ntiles,
+                                                         const int64_t* __restrict__ Ts,
+                                                         unsigned* __restrict__ keys,
+                                                         int32_t* __restrict__ idx) {
+  __shared__ int cur[kCoarse];
+  cur[threadIdx.x] = (int)Ts[(int64_t)threadIdx.x * ntiles + blockIdx.x];
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kSortTile;
+  for (int j0 = 0; j0 < kSortItems; j0 += kSortBatch) {
+    unsigned k[kSortBatch];
+#pragma unroll
+    for (int j = 0; j < kSortBatch; ++j) {
+      const int64_t i = base + (int64_t)(j0 + j) * kBlock + threadIdx.x;
+      k[j] = (i < n) ? order_key(kind, q, i, g) : ~0u;
+    }
+#pragma unroll
+    for (int j = 0; j < kSortBatch; ++j) {
+      if (k[j] != ~0u) {
+        const int pos = atomicAdd(&cur[k[j] >> kFineBits], 1);
+        keys[pos] = k[j];
+        idx[pos] = (int32_t)(base + (int64_t)(j0 + j) * kBlock + threadIdx.x);
+      }
+    }
+  }
 }
 
-__global__ void __launch_bounds__(kBlock) k_order_scatter(int kind, const double* __restrict__ q, int64_t n,
-                                                          OrderGeom g,
-                                                          unsigned long long* __restrict__ cursor,
+__global__ void __launch_bounds__(kFineBlock) k_sort_fine(int64_t ntiles, const int64_t* __restrict__ Ts,
+                                                          int64_t n, const unsigned* __restrict__ keys,
+                                                          const int32_t* __restrict__ idx,
                                                           int32_t* __restrict__ order) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const unsigned long long pos = atomicAdd(cursor + order_key(kind, q, i, g), 1ull);
-  order[pos] = (int32_t)i;
+  __shared__ int hist[kFine];
+  __shared__ int lds[kFineBlock / 64];
+  const int cb = blockIdx.x;
+  const int lo = (int)Ts[(int64_t)cb * ntiles];
+  const int hi = (cb + 1 < kCoarse) ? (int)Ts[(int64_t)(cb + 1) * ntiles] : (int)n;
+  for (int b = threadIdx.x; b < kFine; b += kFineBlock) hist[b] = 0;
+  __syncthreads();
+  constexpr int B = 4;
+  for (int p0 = lo + threadIdx.x; p0 < hi; p0 += B * kFineBlock) {
+    unsigned k[B];
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+      const int p = p0 + j * kFineBlock;
+      k[j] = (p < hi) ? keys[p] : ~0u;
+    }
+#pragma unroll
+    for (int j = 0; j < B; ++j)
+      if (k[j] != ~0u) atomicAdd(&hist[k[j] & (kFine - 1)], 1);
+  }
+  __syncthreads();
+  constexpr int per = kFine / kFineBlock;
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < per; ++k) s += hist[threadIdx.x * per + k];
+  int total;
+  int run = lo + block_exclusive_scan<int, kFineBlock>(s, lds, &total);
+#pragma unroll
+  for (int k = 0; k < per; ++k) {
+    const int c = hist[threadIdx.x * per + k];
+    hist[threadIdx.x * per + k] = run;
+    run += c;
+  }
+  __syncthreads();
+  for (int p0 = lo + threadIdx.x; p0 < hi; p0 += B * kFineBlock) {
+    unsigned k[B];
+    int v[B];
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+      const int p = p0 + j * kFineBlock;
+      k[j] = (p < hi) ? keys[p] : ~0u;
+      v[j] = (p < hi) ? idx[p] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+      if (k[j] != ~0u) {
+        const int pos = atomicAdd(&hist[k[j] & (kFine - 1)], 1);
+        order[pos] = v[j];
+      }
+    }
+  }
 }
 
 struct OrderWs {
-  unsigned* hist;
-  int64_t* wide;
-  unsigned long long* cursor;
+  int64_t* T;
+  int64_t* Ts;
   void* scan_ws;
   size_t scan_bytes;
+  unsigned* keys;
+  int32_t* idx;
   int32_t* order;  // only used by ct_locate_points_ordered
+  int64_t ntiles;
 };
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
-OrderWs carve_order(char* base, int64_t n_order, size_t* total) {
+OrderWs carve_order(char* base, int64_t n, bool with_order, size_t* total) {
   size_t off = 0;
   OrderWs w{};
   auto take = [&](size_t bytes) {
@@ -260,12 +372,16 @@ OrderWs carve_order(char* base, int64_t n_order, size_t* total) {
     off = align256(off + bytes);
     return p;
   };
-  w.hist = reinterpret_cast<unsigned*>(take(sizeof(unsigned) * kBuckets));
-  w.wide = reinterpret_cast<int64_t*>(take(sizeof(int64_t) * kBuckets));
-  w.cursor = reinterpret_cast<unsigned long long*>(take(sizeof(unsigned long long) * (kBuckets + 1)));
-  w.scan_bytes = ct_scan_workspace_bytes(kBuckets);
+  const int64_t nn = n > 0 ? n : 1;
+  w.ntiles = (nn + kSortTile - 1) / kSortTile;
+  const int64_t nt = (int64_t)kCoarse * w.ntiles;
+  w.T = reinterpret_cast<int64_t*>(take(sizeof(int64_t) * nt));
+  w.Ts = reinterpret_cast<int64_t*>(take(sizeof(int64_t) * (nt + 1)));
+  w.scan_bytes = ct_scan_workspace_bytes(nt);
   w.scan_ws = take(w.scan_bytes);
-  w.order = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * (size_t)(n_order > 0 ? n_order : 1)));
+  w.keys = reinterpret_cast<unsigned*>(take(sizeof(unsigned) * nn));
+  w.idx = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * nn));
+  w.order = with_order ? reinterpret_cast<int32_t*>(take(sizeof(int32_t) * nn)) : nullptr;
   *total = off + 256;
   return w;
 }
@@ -278,15 +394,14 @@ int query_order(int kind, const double* q, int64_t n, const double* bbox, const 
   g.sx = 1.0 / ex;
   g.sy = 1.0 / ey;
   g.inv_area = 1.0 / (ex * ey);
-  int rc = fill<unsigned>(w.hist, kBuckets, 0u, st);
-  if (rc) return rc;
-  k_order_hist<<<grid_for(n), kBlock, 0, st>>>(kind, q, n, g, w.hist);
+  const unsigned ntiles = (unsigned)w.ntiles;
+  k_sort_hist<<<ntiles, kBlock, 0, st>>>(kind, q, n, g, w.ntiles, w.T);
   CT_LAUNCH_CHECK();
-  k_order_widen<<<grid_for(kBuckets), kBlock, 0, st>>>(w.hist, kBuckets, w.wide);
-  CT_LAUNCH_CHECK();
-  rc = exclusive_scan(w.wide, kBuckets, reinterpret_cast<int64_t*>(w.cursor), w.scan_ws, w.scan_bytes, st);
+  int rc = exclusive_scan(w.T, (int64_t)kCoarse * w.ntiles, w.Ts, w.scan_ws, w.scan_bytes, st);
   if (rc) return rc;
-  k_order_scatter<<<grid_for(n), kBlock, 0, st>>>(kind, q, n, g, w.cursor, order);
+  k_sort_scatter<<<ntiles, kBlock, 0, st>>>(kind, q, n, g, w.ntiles, w.Ts, w.keys, w.idx);
+  CT_LAUNCH_CHECK();
+  k_sort_fine<<<kCoarse, kFineBlock, 0, st>>>(w.ntiles, w.Ts, n, w.keys, w.idx, order);
   CT_LAUNCH_CHECK();
   return CT_OK;
 }
@@ -711,7 +826,7 @@ struct LocateLaunch {
   hipStream_t st;
   template <int S>
   int run() {
-    k_locate_points<MAXV, S, ON_EDGE, PACKED><<<grid_for(n), kBlock, 0, st>>>(t, pts, order, n, tol, out);
+    k_locate_points<MAXV, S, ON_EDGE, PACKED><<<grid_xcd(n), kBlock, 0, st>>>(t, pts, order, n, tol, out);
     CT_LAUNCH_CHECK();
     return CT_OK;
   }
@@ -853,9 +968,8 @@ int ct_pack_leaf_coords(const ct_tree* tree, double* out, void* stream) {
 }
 
 size_t ct_order_workspace_bytes(int64_t n) {
-  (void)n;
   size_t total = 0;
-  carve_order(nullptr, 0, &total);
+  carve_order(nullptr, n, false, &total);
   return total;
 }
 
@@ -865,7 +979,7 @@ int ct_query_order(int32_t kind, const double* queries, int64_t n, const double*
              "bad arguments");
   if (n == 0) return CT_OK;
   size_t need = 0;
-  OrderWs w = carve_order(static_cast<char*>(workspace), 0, &need);
+  OrderWs w = carve_order(static_cast<char*>(workspace), n, false, &need);
   if (workspace_bytes < need) {
     set_error("order workspace too small: %zu < %zu", workspace_bytes, need);
     return CT_ERR_WORKSPACE;
@@ -875,7 +989,7 @@ int ct_query_order(int32_t kind, const double* queries, int64_t n, const double*
 
 size_t ct_locate_workspace_bytes(int64_t n) {
   size_t total = 0;
-  carve_order(nullptr, n < 0 ? 0 : n, &total);
+  carve_order(nullptr, n, true, &total);
   return total;
 }
 
@@ -886,7 +1000,7 @@ int ct_locate_points_ordered(const ct_tree* tree, const double* points, int64_t 
   CT_REQUIRE(n >= 0 && n < ((int64_t)1 << 31) && tree->n_max_vert >= 3, "bad arguments");
   if (n == 0) return CT_OK;
   size_t need = 0;
-  OrderWs w = carve_order(static_cast<char*>(workspace), n, &need);
+  OrderWs w = carve_order(static_cast<char*>(workspace), n, true, &need);
   if (workspace_bytes < need) {
     set_error("locate workspace too small: %zu < %zu", workspace_bytes, need);
     return CT_ERR_WORKSPACE;
@@ -895,6 +1009,15 @@ int ct_locate_points_ordered(const ct_tree* tree, const double* points, int64_t 
   rc = query_order(0, points, n, tree->bbox, w, w.order, st);
   if (rc) return rc;
   return launch_locate_faces(tree, points, w.order, n, tol, out, st);
+}
+
+int ct_locate_points_in_order(const ct_tree* tree, const double* points, int64_t n,
+                              const int32_t* order, double tol, int64_t* out, void* stream) {
+  int rc = check_tree(tree);
+  if (rc) return rc;
+  CT_REQUIRE(n >= 0 && n < ((int64_t)1 << 31) && tree->n_max_vert >= 3, "bad arguments");
+  if (n == 0) return CT_OK;
+  return launch_locate_faces(tree, points, order, n, tol, out, as_stream(stream));
 }
 
 int ct_locate_points_on_edge(const ct_tree* tree, const double* points, int64_t n, double tol,

@@ -132,6 +132,31 @@ def test_locate_points_every_path(trees):
         g._ctree.leaf_coords = saved
 
 
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_query_order_is_permutation(kind):
+    """ct_query_order returns a permutation, also for clustered, outside and NaN queries."""
+    import torch
+
+    from numba_celltree_amd import _device as D
+
+    rng = np.random.default_rng(kind)
+    n = 300_001
+    if kind == 0:
+        q = rng.uniform(-10, 110, size=(n, 2))
+        q[:1000] = 50.0  # one bucket
+        q[1000] = np.nan
+    elif kind == 1:
+        c = rng.uniform(0, 100, size=(n, 2))
+        w = np.exp(rng.uniform(-5, 5, size=(n, 2)))
+        q = np.column_stack([c[:, 0] - w[:, 0], c[:, 0] + w[:, 0], c[:, 1] - w[:, 1], c[:, 1] + w[:, 1]])
+        q[7] = np.inf
+    else:
+        q = S.segments(n, 100.0, 1.0, 0.01, 100.0, 3)
+    order = D.query_order(kind, torch.from_numpy(np.ascontiguousarray(q)).cuda(), [0.0, 100.0, 0.0, 100.0])
+    o = order.cpu().numpy()
+    assert np.array_equal(np.sort(o), np.arange(n))
+
+
 def test_locate_boxes_ordered(trees):
     name, v, f, g, o = trees
     L = _extent(v)
