@@ -289,6 +289,14 @@ int ct_regrid_vertex_backward(const double* tgt_vertices, const int32_t* tgt_fac
                               const double* tgt_area, int32_t n_fields, double* g_tgt_vertices,
                               double* g_src_vertices, void* stream);
 
+/* Adjoint of barycentric_triangle.py:28-64 with each point's face held fixed: for dL/dw
+ * (n, 3) writes dL/dpoints (n, 2) (NULL to skip) and adds dL/dvertices (NULL to skip; the
+ * caller zeroes it).  Points with face -1 get zero gradient. */
+int ct_barycentric_triangle_backward(const double* points, const int64_t* face_indices, int64_t n,
+                                     const int32_t* faces, int32_t n_max_vert,
+                                     const double* vertices, const double* g_weights,
+                                     double* g_points, double* g_vertices, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

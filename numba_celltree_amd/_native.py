@@ -107,6 +107,7 @@ SIGNATURES = {
     "ct_polygon_areas": (ctypes.c_int, [_P, _P, _I32, _I64, _P, _P]),
     "ct_regrid_apply": (ctypes.c_int, [_P, _P, _P, _P, _P, _I32, _I64, _P, _P]),
     "ct_regrid_apply_transpose": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I32, _I64, _P, _P]),
+    "ct_barycentric_triangle_backward": (ctypes.c_int, [_P, _P, _I64, _P, _I32, _P, _P, _P, _P, _P]),
     "ct_regrid_vertex_backward": (
         ctypes.c_int,
         [_P, _P, _I32, _P, _P, _I32, _P, _P, _I64, _I64, _P, _P, _P, _P, _I32, _P, _P, _P],

@@ -1,0 +1,53 @@
+"""Differentiable barycentric weights (no reference counterpart for the adjoint; the forward
+is barycentric_triangle.py:28-64 as used by celltree.py:306-351).
+
+Each point's containing face is located once (not differentiable) and held fixed; the
+weights are then differentiable with respect to the points and the mesh vertices, with the
+hand-written adjoint ct_barycentric_triangle_backward.
+"""
+
+from typing import Optional
+
+import torch
+
+from numba_celltree_amd import _device as D
+from numba_celltree_amd import _native
+
+
+class BarycentricTriangleFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, points, vertices, faces, face_index, tolerance: float):
+        points = points.contiguous()
+        vertices = vertices.contiguous()
+        n = int(points.shape[0])
+        w = D.empty((n, 3), torch.float64)
+        _native.call("ct_barycentric_triangle_weights", D.ptr(points), D.ptr(face_index), n, D.ptr(faces),
+                     int(faces.shape[1]), D.ptr(vertices), float(tolerance), D.ptr(w), D.stream())
+        ctx.save_for_backward(points, vertices, faces, face_index)
+        return w
+
+    @staticmethod
+    def backward(ctx, gw):
+        points, vertices, faces, face_index = ctx.saved_tensors
+        n = int(points.shape[0])
+        gp = D.empty((n, 2), torch.float64) if ctx.needs_input_grad[0] else None
+        gv = torch.zeros_like(vertices) if ctx.needs_input_grad[1] else None
+        _native.call("ct_barycentric_triangle_backward", D.ptr(points), D.ptr(face_index), n, D.ptr(faces),
+                     int(faces.shape[1]), D.ptr(vertices), D.ptr(gw.contiguous()), D.ptr(gp), D.ptr(gv),
+                     D.stream())
+        return gp, gv, None, None, None
+
+
+def barycentric_weights(tree, points: torch.Tensor, vertices: Optional[torch.Tensor] = None,
+                        tolerance: Optional[float] = None):
+    """(face_index, weights) like ``CellTree2d.compute_barycentric_weights`` on a triangle
+    tree, with weights differentiable in ``points`` and ``vertices`` (faces held fixed)."""
+    if int(tree.d_elements.shape[1]) != 3:
+        raise NotImplementedError("the barycentric adjoint covers triangle meshes")
+    if tolerance is None:
+        tolerance = tree._default_tolerance()
+    if vertices is None:
+        vertices = tree.d_vertices
+    face = tree._locate_points_dev(points.detach().contiguous(), float(tolerance))
+    w = BarycentricTriangleFunction.apply(points, vertices, tree.d_elements, face, float(tolerance))
+    return face, w

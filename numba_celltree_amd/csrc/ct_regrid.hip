@@ -381,6 +381,63 @@ __global__ void __launch_bounds__(kBlock) k_apply_t(const int64_t* __restrict__ 
   gF[i] = acc;
 }
 
+// Adjoint of barycentric_triangle.py:28-43 compute_weights (face held fixed):
+//   ab = b - a, ac = c - a, ap = p - a; Aa = |ab x ap|, Ac = |ac x ap|, A = |ab x ac|
+//   w = Aa / A (as (1/A) * Aa), v = Ac / A, u = 1 - v - w.
+__global__ void __launch_bounds__(kBlock) k_bary_tri_backward(const double* __restrict__ pts,
+                                                              const int64_t* __restrict__ face, int64_t n,
+                                                              const int32_t* __restrict__ faces, int nmax,
+                                                              const double* __restrict__ vtx,
+                                                              const double* __restrict__ gw,
+                                                              double* __restrict__ g_pts,
+                                                              double* __restrict__ g_vtx) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t f = face[i];
+  if (f < 0) {
+    if (g_pts) {
+      g_pts[2 * i] = 0.0;
+      g_pts[2 * i + 1] = 0.0;
+    }
+    return;
+  }
+  const int32_t* row = faces + f * nmax;
+  const int ia = row[0], ib = row[1], ic = row[2];
+  const P2 a = ld_vertex(vtx, ia), b = ld_vertex(vtx, ib), c = ld_vertex(vtx, ic);
+  const P2 p = {pts[2 * i], pts[2 * i + 1]};
+  const double abx = b.x - a.x, aby = b.y - a.y;
+  const double acx = c.x - a.x, acy = c.y - a.y;
+  const double apx = p.x - a.x, apy = p.y - a.y;
+  const double cAa = abx * apy - aby * apx;
+  const double cAc = acx * apy - acy * apx;
+  const double cA = abx * acy - aby * acx;
+  const double Aa = fabs(cAa), Ac = fabs(cAc), A = fabs(cA);
+  const double inv = 1.0 / A;
+  const double gu = gw[3 * i], gv = gw[3 * i + 1], gww = gw[3 * i + 2];
+  const double gv_t = gv - gu, gw_t = gww - gu;  // u = 1 - v - w
+  const double g_inv = gw_t * Aa + gv_t * Ac;
+  const double gAa = gw_t * inv, gAc = gv_t * inv;
+  const double gA = -g_inv * inv * inv;
+  auto sgn = [](double x) { return x > 0 ? 1.0 : (x < 0 ? -1.0 : 0.0); };
+  const double g1 = gAa * sgn(cAa), g2 = gAc * sgn(cAc), g3 = gA * sgn(cA);
+  // cross(u, v) = u.x v.y - u.y v.x: d/du = (v.y, -v.x), d/dv = (-u.y, u.x)
+  double gabx = g1 * apy + g3 * acy, gaby = -g1 * apx - g3 * acx;
+  double gacx = g2 * apy - g3 * aby, gacy = -g2 * apx + g3 * abx;
+  double gapx = -g1 * aby - g2 * acy, gapy = g1 * abx + g2 * acx;
+  if (g_pts) {
+    g_pts[2 * i] = gapx;
+    g_pts[2 * i + 1] = gapy;
+  }
+  if (g_vtx) {
+    atomicAdd(g_vtx + 2 * ib, gabx);
+    atomicAdd(g_vtx + 2 * ib + 1, gaby);
+    atomicAdd(g_vtx + 2 * ic, gacx);
+    atomicAdd(g_vtx + 2 * ic + 1, gacy);
+    atomicAdd(g_vtx + 2 * ia, -(gabx + gacx + gapx));
+    atomicAdd(g_vtx + 2 * ia + 1, -(gaby + gacy + gapy));
+  }
+}
+
 inline int vclass(int nmax) { return nmax <= 4 ? 4 : nmax <= 8 ? 8 : 16; }
 
 #define CT_PAIR_DISPATCH(K, A, B, ...)                                                       \
@@ -444,6 +501,18 @@ int ct_regrid_apply_transpose(const int64_t* src_offsets, const int64_t* perm, c
   if (n_src == 0) return CT_OK;
   k_apply_t<<<grid_for(n_src * n_fields), kBlock, 0, as_stream(stream)>>>(src_offsets, perm, tt, area, tgt_area,
                                                                           gy, n_fields, n_src, gF);
+  CT_LAUNCH_CHECK();
+  return CT_OK;
+}
+
+int ct_barycentric_triangle_backward(const double* points, const int64_t* face_indices, int64_t n,
+                                     const int32_t* faces, int32_t n_max_vert, const double* vertices,
+                                     const double* g_weights, double* g_points, double* g_vertices,
+                                     void* stream) {
+  CT_REQUIRE(n >= 0 && n_max_vert >= 3, "bad sizes");
+  if (n == 0) return CT_OK;
+  k_bary_tri_backward<<<grid_for(n), kBlock, 0, as_stream(stream)>>>(points, face_indices, n, faces, n_max_vert,
+                                                                     vertices, g_weights, g_points, g_vertices);
   CT_LAUNCH_CHECK();
   return CT_OK;
 }

@@ -204,3 +204,52 @@ def test_matches_torch_autograd_restatement():
         b = b.numpy()
         scale = np.abs(b).max()
         assert np.allclose(a, b, rtol=1e-10, atol=1e-12 * scale), name
+
+
+# ------------------------------------------------------------------ barycentric adjoint
+def test_barycentric_adjoint():
+    """Triangle weights with faces held fixed: forward equals compute_barycentric_weights;
+    gradients in points and vertices match central finite differences within 1e-8."""
+    import torch
+
+    import numba_celltree_amd as M
+    from numba_celltree_amd.barycentric import barycentric_weights
+
+    v, f, _ = S.config1()
+    tree = M.CellTree2d(v, f, -1)
+    rng = np.random.default_rng(4)
+    pts = torch.from_numpy(rng.uniform(0.2, 9.8, size=(20_000, 2))).cuda()
+    face, w = barycentric_weights(tree, pts)
+    ref_face, ref_w = tree.compute_barycentric_weights(pts)
+    assert torch.equal(face, ref_face) and torch.equal(w, ref_w)
+    G = torch.from_numpy(rng.normal(size=(len(pts), 3))).cuda()
+    p = pts.clone().requires_grad_(True)
+    vv = tree.d_vertices.clone().requires_grad_(True)
+    _, w = barycentric_weights(tree, p, vv)
+    (w * G).sum().backward()
+    from numba_celltree_amd.barycentric import BarycentricTriangleFunction
+
+    tol = tree._default_tolerance()
+
+    def loss(px, vx):
+        with torch.no_grad():
+            return float((BarycentricTriangleFunction.apply(px, vx, tree.d_elements, face, tol) * G).sum())
+
+    h = 1e-7
+    for name, base, grad, k in (("points", pts, p.grad, 0), ("vertices", tree.d_vertices, vv.grad, 1)):
+        d = torch.from_numpy(rng.normal(size=tuple(base.shape))).cuda()
+        args_p = [pts, tree.d_vertices]
+        args_m = [pts, tree.d_vertices]
+        args_p[k] = base + h * d
+        args_m[k] = base - h * d
+        fd = (loss(*args_p) - loss(*args_m)) / (2 * h)
+        an = float((grad * d).sum())
+        rel = abs(fd - an) / max(abs(an), abs(fd))
+        print(f"[bary fd] {name}: analytic {an:.15e} fd {fd:.15e} rel {rel:.2e}")
+        assert rel < 1e-8, (name, an, fd, rel)
+    small = pts[:50].clone().requires_grad_(True)
+    vs = tree.d_vertices.clone().requires_grad_(True)
+    assert torch.autograd.gradcheck(
+        lambda a, b: BarycentricTriangleFunction.apply(a, b, tree.d_elements, face[:50], tol), (small, vs),
+        eps=1e-6, atol=1e-8, rtol=1e-6,
+    )
