@@ -161,6 +161,83 @@ def extras(args):
     return out
 
 
+def batch_extras(args):
+    """512 independent meshes, each with its own points and segments: a loop of plain
+    per-mesh calls against one HIP-graph replay of the batched persistent kernels."""
+    import torch
+
+    import numba_celltree_amd as M
+    from numba_celltree_amd import batch as B
+    from numba_celltree_amd import synthetic as S
+
+    out = {}
+    t0 = time.perf_counter()
+    sets = S.config5(n_meshes=args.batch_meshes)
+    log(f"batch workload generated in {time.perf_counter() - t0:.1f} s")
+    dev = torch.device("cuda")
+    t0 = time.perf_counter()
+    trees = [M.CellTree2d(torch.from_numpy(v).to(dev), torch.from_numpy(f).to(dev), -1) for v, f, _, _ in sets]
+    torch.cuda.synchronize()
+    out["batch_build_trees_ms"] = 1e3 * (time.perf_counter() - t0)
+    d_pts = [torch.from_numpy(p).to(dev) for _, _, p, _ in sets]
+    d_seg = [torch.from_numpy(s).to(dev) for _, _, _, s in sets]
+    out["batch_meshes"] = len(sets)
+    out["batch_cells"] = int(sum(len(f) for _, f, _, _ in sets))
+    out["batch_points"] = int(sum(len(p) for _, _, p, _ in sets))
+    out["batch_segments"] = int(sum(len(s) for _, _, _, s in sets))
+
+    def plain_points():
+        return [t.locate_points(p) for t, p in zip(trees, d_pts)]
+
+    def plain_edges():
+        return [t.intersect_edges(s) for t, s in zip(trees, d_seg)]
+
+    def plain_both():
+        return plain_points(), plain_edges()
+
+    for name, fn in (("batch_plain_points_ms", plain_points), ("batch_plain_edges_ms", plain_edges),
+                     ("batch_plain_ms", plain_both)):
+        fn()
+        wall, _ = time_calls(fn, 3)
+        out[name] = 1e3 * wall / 3
+    want_pts = plain_points()
+    want_edges = plain_edges()
+
+    tb = B.TreeBatch(trees)
+    pts, pc = B.concat_queries(d_pts, 2)
+    segs, sc = B.concat_queries(d_seg, 4)
+    segs = segs.reshape(-1, 2, 2)
+    t0 = time.perf_counter()
+    cfg_p, tp = B.autotune(tb, "points", pts, pc, reps=2)
+    cfg_e, te = B.autotune(tb, "edges", segs, sc, reps=2)
+    out["batch_autotune_s"] = time.perf_counter() - t0
+    out["batch_config_points"] = cfg_p.__dict__
+    out["batch_config_edges"] = cfg_e.__dict__
+    out["batch_autotune_points_ms"] = {"best": min(ms for _, ms in tp), "worst": max(ms for _, ms in tp)}
+    out["batch_autotune_edges_count_ms"] = {"best": min(ms for _, ms in te), "worst": max(ms for _, ms in te)}
+    graphs = {
+        "batch_graph_points_ms": B.BatchGraph(tb, pts, pc, points_config=cfg_p),
+        "batch_graph_edges_ms": B.BatchGraph(tb, segments=segs, segment_counts=sc, edges_config=cfg_e),
+        "batch_graph_ms": B.BatchGraph(tb, pts, pc, segs, sc, points_config=cfg_p, edges_config=cfg_e),
+    }
+    for name, g in graphs.items():
+        g.replay()
+        wall, _ = time_calls(g.replay, 5)
+        out[name] = 1e3 * wall / 5
+    res = graphs["batch_graph_ms"].result()
+    ro = res.row_offsets()
+    ok = all(torch.equal(res.points_of(m), want_pts[m]) for m in range(len(trees)))
+    for m in range(len(trees)):
+        gi, gj, gxy = res.edges_of(m, ro)
+        wi, wj, wxy = want_edges[m]
+        ok = ok and torch.equal(gi, wi) and torch.equal(gj, wj) and torch.equal(gxy, wxy)
+    out["batch_identical_to_plain"] = bool(ok)
+    out["batch_speedup_points"] = out["batch_plain_points_ms"] / out["batch_graph_points_ms"]
+    out["batch_speedup_edges"] = out["batch_plain_edges_ms"] / out["batch_graph_edges_ms"]
+    out["batch_speedup"] = out["batch_plain_ms"] / out["batch_graph_ms"]
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,8 +248,11 @@ def main() -> int:
     ap.add_argument("--segments", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--only", choices=["all", "locate", "extras"], default="all",
-                    help="locate: the headline workload only; extras: the secondary workloads only")
+    ap.add_argument("--batch-meshes", type=int, default=512)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--only", choices=["all", "locate", "extras", "batch"], default="all",
+                    help="locate: the headline workload only; extras: the secondary workloads only; "
+                         "batch: the many-mesh batch only")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run three 1 GiB device copies and exit (counter calibration)")
     args = ap.parse_args()
@@ -202,6 +282,9 @@ def main() -> int:
         return 0
     if args.only == "extras":
         print(json.dumps({"extras": extras(args)}), flush=True)
+        return 0
+    if args.only == "batch":
+        print(json.dumps({"batch": batch_extras(args)}), flush=True)
         return 0
     log("generating workload")
     v, f = S.config2_mesh()
@@ -311,6 +394,10 @@ def main() -> int:
         del tree
         torch.cuda.empty_cache()
         result["extras"] = extras(args)
+        if not args.no_batch:
+            log("batch")
+            torch.cuda.empty_cache()
+            result["batch"] = batch_extras(args)
     print(json.dumps(result), flush=True)
     return 0
 

@@ -61,6 +61,8 @@ typedef struct ct_tree {
   const double* leaf_coords; /* optional (may be NULL): (n_elements, n_max_vert, 2), row i =
                                 the vertices of cell bb_indices[i], unused slots NaN; see
                                 ct_pack_leaf_coords */
+  const double* node_boxes;  /* optional (may be NULL): (n_nodes + 1, 4), row j + 1 = the
+                                tight box of node j's cells; see ct_node_boxes */
 } ct_tree;
 
 const char* ct_last_error(void);
@@ -126,6 +128,14 @@ int ct_locate_points(const ct_tree* tree, const double* points, int64_t n, doubl
  * a leaf's cells contiguously instead of through bb_indices -> faces -> vertices.  Requires
  * 3 <= n_max_vert <= 8 and non-NaN vertex coordinates. */
 int ct_pack_leaf_coords(const ct_tree* tree, double* out, void* stream);
+
+/* Layout helper, no reference counterpart: out (n_nodes + 1, 4) gets, at row j + 1, the
+ * tight box (xmin, xmax, ymin, ymax) of node j's cells bb_indices[ptr .. ptr + size); any
+ * NaN cell coordinate makes the box NaN.  Segment-face queries (ct_locate_edges_*, kind 1)
+ * of a tree with node_boxes set skip subtrees the segment cannot reach; their results are
+ * the same with or without it.  scratch: device int32 array of n_nodes.  Enqueues
+ * 1 + depth kernels. */
+int ct_node_boxes(const ct_tree* tree, int32_t* scratch, double* out, void* stream);
 
 size_t ct_locate_workspace_bytes(int64_t n);
 
@@ -296,6 +306,61 @@ int ct_barycentric_triangle_backward(const double* points, const int64_t* face_i
                                      const int32_t* faces, int32_t n_max_vert,
                                      const double* vertices, const double* g_weights,
                                      double* g_points, double* g_vertices, void* stream);
+
+/* ------------------------------------------------------------------ batched trees */
+/* No reference counterpart: many trees and their query sets in one persistent launch,
+ * replacing a loop of per-mesh calls (celltree.py:216-239 locate_points, celltree.py:294-303
+ * intersect_edges).  Results equal the per-tree calls'.  Every call clears `counters` with
+ * a fill kernel and allocates nothing, so a sequence of batch calls may be captured in a
+ * HIP graph; calls sharing one `counters` array must not overlap. */
+
+/* A tile: up to 64 * k consecutive queries of one tree, taken by one wavefront. */
+typedef struct ct_batch_tile {
+  int64_t begin;  /* first query, relative to the tree's first query */
+  int32_t count;  /* number of queries, >= 1 */
+  int32_t tree;   /* index into ct_batch.trees */
+} ct_batch_tile;
+
+/* The batch description; the struct itself is a HOST value, its arrays are on the device. */
+typedef struct ct_batch {
+  const ct_tree* trees;         /* (n_trees) tree descriptors */
+  const double* tolerances;     /* (n_trees) point-location tolerance per tree */
+  const int64_t* query_offsets; /* (n_trees + 1): tree m owns queries [off[m], off[m + 1]) */
+  const ct_batch_tile* tiles;   /* (n_tiles), grouped by queue */
+  const int32_t* queue_starts;  /* (n_queues + 1): queue r owns tiles [s[r], s[r + 1]) */
+  int32_t* counters;            /* (n_queues) scratch */
+  int64_t n_tiles;
+  int32_t n_trees;
+  int32_t n_queues;             /* 1 .. 64; workgroup b starts on queue b % n_queues */
+  int32_t max_n_vert;           /* largest n_max_vert over the trees, 3 .. 8 (faces) */
+  int32_t max_depth;            /* largest tree depth */
+  int32_t stack;                /* traversal stack class, >= max_depth: 16, 24, 32, 48, 64, 128, 256 */
+  int32_t n_workgroups;         /* persistent grid size */
+  int32_t block;                /* workgroup size: 64, 128, 256 or 512 */
+  int32_t reserved;
+} ct_batch;
+
+/* locate_points for every tree's queries: out[q] = the first containing cell of q's tree
+ * (that tree's cell index), else -1.  points: (n_queries, 2) concatenated. */
+int ct_batch_locate_points(const ct_batch* batch, const double* points, int64_t* out, void* stream);
+
+/* intersect_edges (segment-face) for every tree's segments, first pass: counts[q]. */
+int ct_batch_edges_count(const ct_batch* batch, const double* segments, int64_t* counts, void* stream);
+
+/* Second pass: rows of query q at offsets[q] (ct_exclusive_scan of the counts), written only
+ * when offsets[q + 1] <= capacity; ii = concatenated query index, jj = cell of q's tree,
+ * xy (rows, 2, 2).  Rows are in DFS order per query; ct_sort_intersections_by_edge then gives
+ * the reference's order. */
+int ct_batch_edges_write(const ct_batch* batch, const double* segments, const int64_t* offsets,
+                         int64_t capacity, int64_t* ii, int64_t* jj, double* xy, void* stream);
+
+/* geometry_utils.py:564-574 sort_intersections_by_edge over the rows of ct_batch_edges_write
+ * (ii = concatenated segment index), like ct_sort_intersections_by_edge but with the row
+ * count read on the device from offsets[n_segments]: rows sorted only if it is <= capacity.
+ * A fixed grid of n_workgroups, so the call can be captured before the count is known. */
+int ct_batch_sort_intersections(const double* segments, int64_t n_segments, const int64_t* offsets,
+                                const int64_t* ii, const int64_t* jj, const double* xy, int64_t capacity,
+                                int32_t n_workgroups, int64_t* jj_out, double* xy_out, void* stream);
 
 #ifdef __cplusplus
 }

@@ -50,10 +50,41 @@ class CtTree(ctypes.Structure):
         ("depth", ctypes.c_int32),
         ("bbox", ctypes.c_double * 4),
         ("leaf_coords", ctypes.c_void_p),
+        ("node_boxes", ctypes.c_void_p),
+    ]
+
+
+class CtBatchTile(ctypes.Structure):
+    """``ct_batch_tile`` (16 bytes)."""
+
+    _fields_ = [("begin", ctypes.c_int64), ("count", ctypes.c_int32), ("tree", ctypes.c_int32)]
+
+
+class CtBatch(ctypes.Structure):
+    """``ct_batch``: host description of a batch of trees (device arrays + sizes)."""
+
+    _fields_ = [
+        ("trees", ctypes.c_void_p),
+        ("tolerances", ctypes.c_void_p),
+        ("query_offsets", ctypes.c_void_p),
+        ("tiles", ctypes.c_void_p),
+        ("queue_starts", ctypes.c_void_p),
+        ("counters", ctypes.c_void_p),
+        ("n_tiles", ctypes.c_int64),
+        ("n_trees", ctypes.c_int32),
+        ("n_queues", ctypes.c_int32),
+        ("max_n_vert", ctypes.c_int32),
+        ("max_depth", ctypes.c_int32),
+        ("stack", ctypes.c_int32),
+        ("n_workgroups", ctypes.c_int32),
+        ("block", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
 assert ctypes.sizeof(CtNode) == 32
+assert ctypes.sizeof(CtTree) == 120
+assert ctypes.sizeof(CtBatchTile) == 16
 
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
@@ -61,6 +92,7 @@ _I32 = ctypes.c_int32
 _F64 = ctypes.c_double
 _SZ = ctypes.c_size_t
 _TP = ctypes.POINTER(CtTree)
+_BP = ctypes.POINTER(CtBatch)
 
 # name -> (restype, argtypes); mirrors include/celltree_hip.h one to one.
 SIGNATURES = {
@@ -81,6 +113,7 @@ SIGNATURES = {
     "ct_validate_node_bounds": (ctypes.c_int, [_TP, _P, _P, _P]),
     "ct_locate_points": (ctypes.c_int, [_TP, _P, _I64, _F64, _P, _P]),
     "ct_pack_leaf_coords": (ctypes.c_int, [_TP, _P, _P]),
+    "ct_node_boxes": (ctypes.c_int, [_TP, _P, _P, _P]),
     "ct_locate_workspace_bytes": (_SZ, [_I64]),
     "ct_locate_points_ordered": (ctypes.c_int, [_TP, _P, _I64, _F64, _P, _SZ, _P, _P]),
     "ct_locate_points_in_order": (ctypes.c_int, [_TP, _P, _I64, _P, _F64, _P, _P]),
@@ -112,6 +145,10 @@ SIGNATURES = {
         ctypes.c_int,
         [_P, _P, _I32, _P, _P, _I32, _P, _P, _I64, _I64, _P, _P, _P, _P, _I32, _P, _P, _P],
     ),
+    "ct_batch_locate_points": (ctypes.c_int, [_BP, _P, _P, _P]),
+    "ct_batch_edges_count": (ctypes.c_int, [_BP, _P, _P, _P]),
+    "ct_batch_edges_write": (ctypes.c_int, [_BP, _P, _P, _I64, _P, _P, _P, _P]),
+    "ct_batch_sort_intersections": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P, _I64, _I32, _P, _P, _P]),
 }
 
 _lib = None

@@ -87,6 +87,15 @@ class CellTree2dBase:
             self.d_leaf_coords = D.empty((n, nmax, 2), torch.float64)
             _native.call("ct_pack_leaf_coords", self.tree_ref, D.ptr(self.d_leaf_coords), st)
             t.leaf_coords = self.d_leaf_coords.data_ptr()
+        t.node_boxes = None
+        self.d_node_boxes = None
+        if nmax >= 3:
+            # Tight subtree boxes: segment-face queries skip subtrees a segment cannot reach.
+            self.d_node_boxes = D.empty((self.n_nodes + 1, 4), torch.float64)
+            scratch = D.empty(self.n_nodes, torch.int32)
+            _native.call("ct_node_boxes", self.tree_ref, D.ptr(scratch), D.ptr(self.d_node_boxes), st)
+            del scratch
+            t.node_boxes = self.d_node_boxes.data_ptr()
 
     @property
     def tree_ref(self):

@@ -1,12 +1,4 @@
-// ct_query.hip -- tree traversals on the device.
-//
-// One lane per query, depth first.  The reference pushes both viable children and pops the
-// near one at once (query.py:93-105, 236-242, 447-453); here the near child is entered
-// directly and only the far child is deferred on the stack, which visits nodes in exactly
-// the same order with half the stack traffic.  At most one deferred node per tree level is
-// ever pending, so a stack of STACK >= depth entries cannot overflow; STACK is chosen per
-// tree from {16, 24, 32, 48, 64, 128, 256}.  Leaf predicates are the reference's, so
-// first-hit answers and the order of variable-length results match it exactly.
+// ct_query.hip -- single-tree query kernels on the device (traversals in ct_traverse.h).
 //
 // Variable-length queries run twice: a count pass, a device exclusive scan
 // (ct_exclusive_scan), and a write pass at the scanned offsets -- no atomic append; output
@@ -14,132 +6,10 @@
 // (`order`, from ct_query_order): every query writes only its own count / rows.
 #include "ct_internal.h"
 #include "ct_primitives.h"
+#include "ct_traverse.h"
 
 namespace ct {
 namespace {
-
-struct TreeView {
-  const double* vertices;
-  const int32_t* elements;
-  const double* bb;
-  const ct_node* nodes;
-  const int32_t* bb_indices;
-  const double* leaf;  // packed leaf-ordered coordinates, or null
-  int nmax;
-  int n_elements;
-  Box bbox;
-};
-
-TreeView view(const ct_tree* t) {
-  return TreeView{t->vertices, t->elements, t->bb_coords, t->nodes, t->bb_indices, t->leaf_coords,
-                  t->n_max_vert, (int)t->n_elements,
-                  Box{t->bbox[0], t->bbox[1], t->bbox[2], t->bbox[3]}};
-}
-
-// Calls f.template run<S>() with the smallest stack class S >= depth.
-template <typename F>
-int with_stack(int depth, F&& f) {
-  if (depth <= 16) return f.template run<16>();
-  if (depth <= 24) return f.template run<24>();
-  if (depth <= 32) return f.template run<32>();
-  if (depth <= 48) return f.template run<48>();
-  if (depth <= 64) return f.template run<64>();
-  if (depth <= 128) return f.template run<128>();
-  return f.template run<256>();
-}
-
-struct NodeLd {
-  double Lmax, Rmin;
-  int child, ptr, size, dim;
-};
-
-// A node is 32 bytes, 32-byte aligned: two 16-byte loads.
-__device__ __forceinline__ NodeLd ld_node(const ct_node* __restrict__ nodes, int i) {
-  const double2 a = reinterpret_cast<const double2*>(nodes)[2 * i];
-  const int4 b = reinterpret_cast<const int4*>(nodes)[2 * i + 1];
-  return NodeLd{a.x, a.y, b.x, b.y, b.z, b.w};
-}
-
-__device__ __forceinline__ P2 ld_point(const double* __restrict__ pts, int64_t i) {
-  const double2 d = reinterpret_cast<const double2*>(pts)[i];
-  return {d.x, d.y};
-}
-
-__device__ __forceinline__ int64_t query_index(const int32_t* __restrict__ order, int64_t i) {
-  return order ? (int64_t)order[i] : i;
-}
-
-// Row k of the packed leaf coordinates: the cell's vertices, unused slots NaN.  The length
-// rule is polygon_length's (geometry_utils.py:72-79): the first empty slot at index >= 3.
-template <int MAXV>
-__device__ __forceinline__ int load_leaf_poly(const double* __restrict__ leaf, int nmax, int k,
-                                              P2 (&poly)[MAXV]) {
-  const double2* row = reinterpret_cast<const double2*>(leaf) + (int64_t)k * nmax;
-#pragma unroll
-  for (int i = 0; i < MAXV; ++i) {
-    if (i < nmax) {
-      const double2 d = row[i];
-      poly[i] = {d.x, d.y};
-    } else {
-      poly[i] = {0.0, 0.0};
-    }
-  }
-  int len = nmax < MAXV ? nmax : MAXV;
-#pragma unroll
-  for (int i = MAXV - 1; i >= 3; --i)
-    if (i < nmax && isnan(poly[i].x)) len = i;
-  return len;
-}
-
-// ----------------------------------------------------------------- points
-// query.py:62-107 locate_point (faces) / :120-164 locate_point_on_edge (edges).
-template <int MAXV, int STACK, bool ON_EDGE, bool PACKED>
-__device__ __forceinline__ int locate_one(const TreeView& t, P2 p, double tol) {
-  int stack[STACK];
-  int sp = 0;
-  int node = 0;
-  while (true) {
-    const NodeLd nd = ld_node(t.nodes, node);
-    if (nd.child == -1) {
-      for (int k = nd.ptr; k < nd.ptr + nd.size; ++k) {
-        if constexpr (ON_EDGE) {
-          const int c = t.bb_indices[k];
-          const int2 e = reinterpret_cast<const int2*>(t.elements)[c];
-          if (point_on_edge(p, ld_vertex(t.vertices, e.x), ld_vertex(t.vertices, e.y), tol)) return c;
-        } else if constexpr (PACKED) {
-          P2 poly[MAXV];
-          const int len = load_leaf_poly<MAXV>(t.leaf, t.nmax, k, poly);
-          if (point_in_poly_or_edge<MAXV>(p, poly, len, tol)) return t.bb_indices[k];
-        } else {
-          const int c = t.bb_indices[k];
-          P2 poly[MAXV];
-          const int len = load_poly<MAXV>(t.elements, t.nmax, c, t.vertices, poly);
-          if (point_in_poly_or_edge<MAXV>(p, poly, len, tol)) return c;
-        }
-      }
-      if (sp == 0) return -1;
-      node = stack[--sp];
-      continue;
-    }
-    const double v = nd.dim ? p.y : p.x;
-    const bool left = v <= nd.Lmax;
-    const bool right = v >= nd.Rmin;
-    const int lc = nd.child, rc = lc + 1;
-    if (left && right) {
-      // Both viable: the side with the larger margin goes first (query.py:93-101).
-      const bool right_first = (nd.Lmax - v) < (v - nd.Rmin);
-      stack[sp++] = right_first ? lc : rc;
-      node = right_first ? rc : lc;
-    } else if (left) {
-      node = lc;
-    } else if (right) {
-      node = rc;
-    } else {
-      if (sp == 0) return -1;
-      node = stack[--sp];
-    }
-  }
-}
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on XCD b % 8).
 // With ordered queries, workgroup b takes tile (b % 8) * (grid / 8) + b / 8, so every XCD
@@ -484,30 +354,6 @@ __global__ void __launch_bounds__(kBlock) k_boxes(TreeView t, const double* __re
 }
 
 // ----------------------------------------------------------------- segments
-// query.py:309-328 compute_edge_face_intersect: Cohen-Sutherland on the cell bbox, then
-// Cyrus-Beck on the cell with tol = max(1e-15, 1e-12 * max(w, h)).
-template <int MAXV>
-__device__ __forceinline__ bool edge_face(const TreeView& t, int c, P2 a, P2 b, P2* cc, P2* dd) {
-  const Box box = ld_box(t.bb, c);
-  if (!cohen_sutherland(a, b, box, cc, dd)) return false;
-  P2 poly[MAXV];
-  const int len = load_poly<MAXV>(t.elements, t.nmax, c, t.vertices, poly);
-  const double tol = pmax(kMinTolerance, kToleranceFactor * pmax(box.xmax - box.xmin, box.ymax - box.ymin));
-  return cyrus_beck<MAXV>(a, b, poly, len, tol, cc, dd);
-}
-
-// query.py:292-306 compute_edge_edge_intersect: (c, c).
-__device__ __forceinline__ bool edge_edge(const TreeView& t, int c, P2 a, P2 b, P2* cc, P2* dd) {
-  const int2 e = reinterpret_cast<const int2*>(t.elements)[c];
-  P2 x;
-  const bool r = lines_intersect(a, b, ld_vertex(t.vertices, e.x), ld_vertex(t.vertices, e.y), &x);
-  *cc = x;
-  *dd = x;
-  return r;
-}
-
-// query.py:357-455 locate_edge: Cohen-Sutherland against the tree bbox, then slab pruning
-// per node (the t-tests of :428-441 always hold once the slab test passed, but are kept).
 template <bool WRITE, int MAXV, int STACK, bool FACES>
 __global__ void __launch_bounds__(kBlock) k_edges(TreeView t, const double* __restrict__ segs,
                                                   const int32_t* __restrict__ order, int64_t n,
@@ -520,79 +366,11 @@ __global__ void __launch_bounds__(kBlock) k_edges(TreeView t, const double* __re
   const int64_t q = query_index(order, i);
   const double4 s4 = reinterpret_cast<const double4*>(segs)[q];
   const P2 a = {s4.x, s4.y}, b = {s4.z, s4.w};
-  int64_t cnt = 0;
-  int64_t o = WRITE ? offsets[q] : 0;
-  P2 c0, d0;
-  if (cohen_sutherland(a, b, t.bbox, &c0, &d0)) {
-    const double vx = b.x - a.x, vy = b.y - a.y;
-    int stack[STACK];
-    int sp = 0;
-    int node = 0;
-    while (true) {
-      const NodeLd nd = ld_node(t.nodes, node);
-      bool pop = true;
-      if (nd.child == -1) {
-        for (int k = nd.ptr; k < nd.ptr + nd.size; ++k) {
-          const int c = t.bb_indices[k];
-          P2 cc, dd;
-          const bool hit = FACES ? edge_face<MAXV>(t, c, a, b, &cc, &dd) : edge_edge(t, c, a, b, &cc, &dd);
-          if (hit) {
-            if (WRITE) {
-              ii[o] = q;
-              jj[o] = c;
-              reinterpret_cast<double4*>(xy)[o] = make_double4(cc.x, cc.y, dd.x, dd.y);
-              ++o;
-            } else {
-              ++cnt;
-            }
-          }
-        }
-      } else {
-        const double dx = nd.dim ? vy : vx;
-        const double an = nd.dim ? a.y : a.x;
-        const double bn = nd.dim ? b.y : b.x;
-        double dl, dr;
-        if (dx > 0.0) {
-          dl = nd.Lmax - an;
-          dr = nd.Rmin - bn;
-        } else {
-          dl = nd.Lmax - bn;
-          dr = nd.Rmin - an;
-        }
-        bool left = dl >= 0.0;
-        bool right = dr <= 0.0;
-        if (dx > 0.0) {
-          if (left) left = (dl / dx) >= 0.0;
-          if (right) right = (dr / dx) <= 1.0;
-        } else if (dx < 0.0) {
-          if (left) left = (1.0 - (dl / dx)) >= 0.0;
-          if (right) right = (1.0 - (dr / dx)) <= 1.0;
-        }
-        if (left && right) {
-          stack[sp++] = nd.child;
-          node = nd.child + 1;
-          pop = false;
-        } else if (left || right) {
-          node = nd.child + (right ? 1 : 0);
-          pop = false;
-        }
-      }
-      if (pop) {
-        if (sp == 0) break;
-        node = stack[--sp];
-      }
-    }
+  if (WRITE) {
+    edge_query<true, MAXV, STACK, FACES>(t, a, b, q, offsets[q], ii, jj, xy);
+  } else {
+    counts[q] = edge_query<false, MAXV, STACK, FACES>(t, a, b, q, 0, nullptr, nullptr, nullptr);
   }
-  if (!WRITE) counts[q] = cnt;
-}
-
-// geometry_utils.py:564-574 sort_intersections_by_edge on grouped rows, np.lexsort's order
-// (stable, NaN last).  One lane per row: the row's rank inside its segment's range = rows
-// with a smaller t plus earlier rows with an equal t; the row is written once, to its rank.
-// t = (c - a) . (b - a) with c the clipped start point, x-term + y-term as the reference's
-// np.sum over the last axis.
-__device__ __forceinline__ bool t_less(double x, double y) {
-  return x < y || (!isnan(x) && isnan(y));
 }
 
 __global__ void __launch_bounds__(kBlock) k_sort_by_edge(const double* __restrict__ segs,
@@ -604,21 +382,7 @@ __global__ void __launch_bounds__(kBlock) k_sort_by_edge(const double* __restric
                                                          double* __restrict__ xy_out) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n_rows) return;
-  const int64_t s = ii[r];
-  const int64_t lo = offsets[s], hi = offsets[s + 1];
-  const double4 s4 = reinterpret_cast<const double4*>(segs)[s];
-  const double abx = s4.z - s4.x, aby = s4.w - s4.y;
-  const double4 mine = reinterpret_cast<const double4*>(xy_in)[r];
-  const double t = (mine.x - s4.x) * abx + (mine.y - s4.y) * aby;
-  int64_t rank = lo;
-  for (int64_t k = lo; k < hi; ++k) {
-    if (k == r) continue;
-    const double2 c = reinterpret_cast<const double2*>(xy_in)[2 * k];
-    const double tk = (c.x - s4.x) * abx + (c.y - s4.y) * aby;
-    if (t_less(tk, t) || (k < r && !t_less(t, tk))) ++rank;
-  }
-  jj_out[rank] = jj_in[r];
-  reinterpret_cast<double4*>(xy_out)[rank] = mine;
+  sort_row(segs, offsets, ii, jj_in, xy_in, r, jj_out, xy_out);
 }
 
 // ----------------------------------------------------------------- barycentric
@@ -750,6 +514,56 @@ __global__ void k_parents(const ct_node* __restrict__ nodes, int64_t n_nodes, in
     parent[c] = (int32_t)x;
     parent[c + 1] = (int32_t)x;
   }
+}
+
+// Tight node boxes, bottom-up in rounds without inter-workgroup synchronisation: the leaf
+// pass reduces every leaf's cell boxes (stamp 0); round r (r = 1 .. depth) forms the union of
+// the two children of every internal node whose children were both stamped before round r
+// (stamp < r) and stamps it r.  A node of height h is finished in round h, so `depth` rounds
+// finish the root; children's rows are read only after the kernel that wrote them has ended.
+// NaN poisons the union.
+constexpr int kUnstamped = 0x7fffffff;
+
+__device__ __forceinline__ double4 box_union(double4 a, double4 b) {
+  const double nan = __builtin_nan("");
+  const bool bad = isnan(a.x) || isnan(a.y) || isnan(a.z) || isnan(a.w) || isnan(b.x) || isnan(b.y) ||
+                   isnan(b.z) || isnan(b.w);
+  if (bad) return make_double4(nan, nan, nan, nan);
+  return make_double4(b.x < a.x ? b.x : a.x, b.y > a.y ? b.y : a.y, b.z < a.z ? b.z : a.z,
+                      b.w > a.w ? b.w : a.w);
+}
+
+__global__ void __launch_bounds__(kBlock) k_node_boxes_leaves(TreeView t, int64_t n_nodes,
+                                                              int32_t* __restrict__ stamp,
+                                                              double* __restrict__ out) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n_nodes) return;
+  const ct_node nd = t.nodes[x];
+  if (nd.child != -1) {
+    stamp[x] = kUnstamped;
+    return;
+  }
+  const double inf = __builtin_inf();
+  double4 u = make_double4(inf, -inf, inf, -inf);
+  for (int k = nd.ptr; k < nd.ptr + nd.size; ++k) {
+    const int c = t.bb_indices[k];
+    u = box_union(u, reinterpret_cast<const double4*>(t.bb)[c]);
+  }
+  reinterpret_cast<double4*>(out)[x + 1] = u;
+  stamp[x] = 0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_node_boxes_round(TreeView t, int64_t n_nodes, int r,
+                                                             int32_t* __restrict__ stamp,
+                                                             double* __restrict__ out) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n_nodes || stamp[x] != kUnstamped) return;
+  const int c = t.nodes[x].child;
+  if (c < 0 || (int64_t)c + 1 >= n_nodes) return;
+  if (stamp[c] >= r || stamp[c + 1] >= r) return;
+  double4* rows = reinterpret_cast<double4*>(out) + 1;
+  rows[x] = box_union(rows[c], rows[c + 1]);
+  stamp[x] = r;
 }
 
 // query.py:568-622 collect_node_bounds: a child copies its parent's bounds and replaces
@@ -1133,6 +947,25 @@ int ct_collect_node_bounds(const ct_node* nodes, int64_t n_nodes, const double* 
   k_node_bounds<<<grid_for(n_nodes), kBlock, 0, st>>>(nodes, n_nodes, parent_scratch, bbox[0], bbox[1],
                                                       bbox[2], bbox[3], out);
   CT_LAUNCH_CHECK();
+  return CT_OK;
+}
+
+int ct_node_boxes(const ct_tree* tree, int32_t* scratch, double* out, void* stream) {
+  int rc = check_tree(tree);
+  if (rc) return rc;
+  CT_REQUIRE(tree->n_nodes >= 1 && tree->bb_coords && scratch && out, "bad arguments");
+  const int64_t n = tree->n_nodes;
+  hipStream_t st = as_stream(stream);
+  const double nan = __builtin_nan("");
+  rc = fill<double>(out, 4, nan, st);  // row 0 (unused)
+  if (rc) return rc;
+  const TreeView t = view(tree);
+  k_node_boxes_leaves<<<grid_for(n), kBlock, 0, st>>>(t, n, scratch, out);
+  CT_LAUNCH_CHECK();
+  for (int r = 1; r <= tree->depth; ++r) {
+    k_node_boxes_round<<<grid_for(n), kBlock, 0, st>>>(t, n, r, scratch, out);
+    CT_LAUNCH_CHECK();
+  }
   return CT_OK;
 }
 

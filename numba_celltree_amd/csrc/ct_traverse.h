@@ -1,0 +1,316 @@
+// ct_traverse.h -- per-query tree traversals shared by the single-tree kernels (ct_query.hip)
+// and the many-tree persistent kernels (ct_batch.hip).
+//
+// One lane per query, depth first.  The reference pushes both viable children and pops the
+// near one at once (query.py:93-105, 236-242, 447-453); here the near child is entered
+// directly and only the far child is deferred on the stack, which visits nodes in exactly
+// the same order with half the stack traffic.  At most one deferred node per tree level is
+// ever pending, so a stack of STACK >= depth entries cannot overflow; STACK is chosen per
+// tree from {16, 24, 32, 48, 64, 128, 256}.  Leaf predicates are the reference's, so
+// first-hit answers and the order of variable-length results match it exactly.
+#pragma once
+
+#include "ct_internal.h"
+
+namespace ct {
+namespace {
+
+struct TreeView {
+  const double* vertices;
+  const int32_t* elements;
+  const double* bb;
+  const ct_node* nodes;
+  const int32_t* bb_indices;
+  const double* leaf;  // packed leaf-ordered coordinates, or null
+  const double* nbox;  // tight node boxes, node j at row j + 1, or null
+  int nmax;
+  int n_elements;
+  Box bbox;
+};
+
+__host__ __device__ __forceinline__ TreeView view(const ct_tree& t) {
+  return TreeView{t.vertices, t.elements, t.bb_coords, t.nodes, t.bb_indices, t.leaf_coords, t.node_boxes,
+                  t.n_max_vert, (int)t.n_elements, Box{t.bbox[0], t.bbox[1], t.bbox[2], t.bbox[3]}};
+}
+
+inline TreeView view(const ct_tree* t) { return view(*t); }
+
+// Calls f.template run<S>() with the smallest stack class S >= depth.
+template <typename F>
+int with_stack(int depth, F&& f) {
+  if (depth <= 16) return f.template run<16>();
+  if (depth <= 24) return f.template run<24>();
+  if (depth <= 32) return f.template run<32>();
+  if (depth <= 48) return f.template run<48>();
+  if (depth <= 64) return f.template run<64>();
+  if (depth <= 128) return f.template run<128>();
+  return f.template run<256>();
+}
+
+struct NodeLd {
+  double Lmax, Rmin;
+  int child, ptr, size, dim;
+};
+
+// A node is 32 bytes, 32-byte aligned: two 16-byte loads.
+__device__ __forceinline__ NodeLd ld_node(const ct_node* __restrict__ nodes, int i) {
+  const double2 a = reinterpret_cast<const double2*>(nodes)[2 * i];
+  const int4 b = reinterpret_cast<const int4*>(nodes)[2 * i + 1];
+  return NodeLd{a.x, a.y, b.x, b.y, b.z, b.w};
+}
+
+__device__ __forceinline__ P2 ld_point(const double* __restrict__ pts, int64_t i) {
+  const double2 d = reinterpret_cast<const double2*>(pts)[i];
+  return {d.x, d.y};
+}
+
+__device__ __forceinline__ int64_t query_index(const int32_t* __restrict__ order, int64_t i) {
+  return order ? (int64_t)order[i] : i;
+}
+
+// Row k of the packed leaf coordinates: the cell's vertices, unused slots NaN.  The length
+// rule is polygon_length's (geometry_utils.py:72-79): the first empty slot at index >= 3.
+template <int MAXV>
+__device__ __forceinline__ int load_leaf_poly(const double* __restrict__ leaf, int nmax, int k,
+                                              P2 (&poly)[MAXV]) {
+  const double2* row = reinterpret_cast<const double2*>(leaf) + (int64_t)k * nmax;
+#pragma unroll
+  for (int i = 0; i < MAXV; ++i) {
+    if (i < nmax) {
+      const double2 d = row[i];
+      poly[i] = {d.x, d.y};
+    } else {
+      poly[i] = {0.0, 0.0};
+    }
+  }
+  int len = nmax < MAXV ? nmax : MAXV;
+#pragma unroll
+  for (int i = MAXV - 1; i >= 3; --i)
+    if (i < nmax && isnan(poly[i].x)) len = i;
+  return len;
+}
+
+// ----------------------------------------------------------------- points
+// query.py:62-107 locate_point (faces) / :120-164 locate_point_on_edge (edges).
+template <int MAXV, int STACK, bool ON_EDGE, bool PACKED>
+__device__ __forceinline__ int locate_one(const TreeView& t, P2 p, double tol) {
+  int stack[STACK];
+  int sp = 0;
+  int node = 0;
+  while (true) {
+    const NodeLd nd = ld_node(t.nodes, node);
+    if (nd.child == -1) {
+      for (int k = nd.ptr; k < nd.ptr + nd.size; ++k) {
+        if constexpr (ON_EDGE) {
+          const int c = t.bb_indices[k];
+          const int2 e = reinterpret_cast<const int2*>(t.elements)[c];
+          if (point_on_edge(p, ld_vertex(t.vertices, e.x), ld_vertex(t.vertices, e.y), tol)) return c;
+        } else if constexpr (PACKED) {
+          P2 poly[MAXV];
+          const int len = load_leaf_poly<MAXV>(t.leaf, t.nmax, k, poly);
+          if (point_in_poly_or_edge<MAXV>(p, poly, len, tol)) return t.bb_indices[k];
+        } else {
+          const int c = t.bb_indices[k];
+          P2 poly[MAXV];
+          const int len = load_poly<MAXV>(t.elements, t.nmax, c, t.vertices, poly);
+          if (point_in_poly_or_edge<MAXV>(p, poly, len, tol)) return c;
+        }
+      }
+      if (sp == 0) return -1;
+      node = stack[--sp];
+      continue;
+    }
+    const double v = nd.dim ? p.y : p.x;
+    const bool left = v <= nd.Lmax;
+    const bool right = v >= nd.Rmin;
+    const int lc = nd.child, rc = lc + 1;
+    if (left && right) {
+      // Both viable: the side with the larger margin goes first (query.py:93-101).
+      const bool right_first = (nd.Lmax - v) < (v - nd.Rmin);
+      stack[sp++] = right_first ? lc : rc;
+      node = right_first ? rc : lc;
+    } else if (left) {
+      node = lc;
+    } else if (right) {
+      node = rc;
+    } else {
+      if (sp == 0) return -1;
+      node = stack[--sp];
+    }
+  }
+}
+
+// ----------------------------------------------------------------- segments
+// query.py:309-328 compute_edge_face_intersect: Cohen-Sutherland on the cell bbox, then
+// Cyrus-Beck on the cell with tol = max(1e-15, 1e-12 * max(w, h)).
+template <int MAXV>
+__device__ __forceinline__ bool edge_face(const TreeView& t, int c, P2 a, P2 b, P2* cc, P2* dd) {
+  const Box box = ld_box(t.bb, c);
+  if (!cohen_sutherland(a, b, box, cc, dd)) return false;
+  P2 poly[MAXV];
+  const int len = load_poly<MAXV>(t.elements, t.nmax, c, t.vertices, poly);
+  const double tol = pmax(kMinTolerance, kToleranceFactor * pmax(box.xmax - box.xmin, box.ymax - box.ymin));
+  return cyrus_beck<MAXV>(a, b, poly, len, tol, cc, dd);
+}
+
+// query.py:292-306 compute_edge_edge_intersect: (c, c).
+__device__ __forceinline__ bool edge_edge(const TreeView& t, int c, P2 a, P2 b, P2* cc, P2* dd) {
+  const int2 e = reinterpret_cast<const int2*>(t.elements)[c];
+  P2 x;
+  const bool r = lines_intersect(a, b, ld_vertex(t.vertices, e.x), ld_vertex(t.vertices, e.y), &x);
+  *cc = x;
+  *dd = x;
+  return r;
+}
+
+// Subtree pruning for segment-face queries (no reference counterpart).  A face is a hit only
+// if Cohen-Sutherland accepts the segment against the face's bbox, and every face bbox of a
+// subtree lies inside the subtree's tight box.  Cohen-Sutherland's clipped points stay
+// within a few ulps of the true segment (each clip interpolates inside [a, b]), so when
+// the segment misses the subtree box grown by m = 1e-9 * (largest coordinate magnitude),
+// no face below can be accepted: skipping the subtree changes no result, only the number
+// of faces tested (O(length) instead of O(length^2) for long segments).  Tested by
+// Liang-Barsky slab clipping of t in [0, 1]; a NaN anywhere makes the test accept.
+__device__ __forceinline__ bool seg_meets_box(P2 a, double dx, double dy, double mseg, double4 nb) {
+  const double mb = pmax(pmax(fabs(nb.x), fabs(nb.y)), pmax(fabs(nb.z), fabs(nb.w)));
+  const double m = 1e-9 * pmax(mseg, mb);
+  const double xlo = nb.x - m, xhi = nb.y + m, ylo = nb.z - m, yhi = nb.w + m;
+  double t0 = 0.0, t1 = 1.0;
+  if (dx == 0.0) {
+    if (a.x < xlo || a.x > xhi) return false;
+  } else {
+    double ta = (xlo - a.x) / dx, tb = (xhi - a.x) / dx;
+    if (dx < 0.0) {
+      const double s = ta;
+      ta = tb;
+      tb = s;
+    }
+    t0 = ta > t0 ? ta : t0;
+    t1 = tb < t1 ? tb : t1;
+  }
+  if (dy == 0.0) {
+    if (a.y < ylo || a.y > yhi) return false;
+  } else {
+    double ta = (ylo - a.y) / dy, tb = (yhi - a.y) / dy;
+    if (dy < 0.0) {
+      const double s = ta;
+      ta = tb;
+      tb = s;
+    }
+    t0 = ta > t0 ? ta : t0;
+    t1 = tb < t1 ? tb : t1;
+  }
+  return !(t0 > t1);
+}
+
+// query.py:357-455 locate_edge for segment q = (a, b): Cohen-Sutherland against the tree
+// bbox, then slab pruning per node (the t-tests of :428-441 always hold once the slab test
+// passed, but are kept).  Returns the number of hits; with WRITE, hit rows go to o, o+1, ...
+// Segment-face queries on a tree with node boxes also skip subtrees by seg_meets_box.
+template <bool WRITE, int MAXV, int STACK, bool FACES>
+__device__ __forceinline__ int64_t edge_query(const TreeView& t, P2 a, P2 b, int64_t q, int64_t o,
+                                              int64_t* __restrict__ ii, int64_t* __restrict__ jj,
+                                              double* __restrict__ xy) {
+  int64_t cnt = 0;
+  P2 c0, d0;
+  if (!cohen_sutherland(a, b, t.bbox, &c0, &d0)) return 0;
+  const double vx = b.x - a.x, vy = b.y - a.y;
+  const double mseg = pmax(pmax(fabs(a.x), fabs(a.y)), pmax(fabs(b.x), fabs(b.y)));
+  // Non-finite segments keep the reference traversal unchanged.
+  const bool prune = FACES && t.nbox != nullptr && isfinite(mseg) && isfinite(vx) && isfinite(vy);
+  const double4* __restrict__ nbox = prune ? reinterpret_cast<const double4*>(t.nbox) + 1 : nullptr;
+  int stack[STACK];
+  int sp = 0;
+  int node = 0;
+  while (true) {
+    const NodeLd nd = ld_node(t.nodes, node);
+    bool pop = true;
+    if (nd.child == -1) {
+      for (int k = nd.ptr; k < nd.ptr + nd.size; ++k) {
+        const int c = t.bb_indices[k];
+        P2 cc, dd;
+        const bool hit = FACES ? edge_face<MAXV>(t, c, a, b, &cc, &dd) : edge_edge(t, c, a, b, &cc, &dd);
+        if (hit) {
+          if (WRITE) {
+            ii[o + cnt] = q;
+            jj[o + cnt] = c;
+            reinterpret_cast<double4*>(xy)[o + cnt] = make_double4(cc.x, cc.y, dd.x, dd.y);
+          }
+          ++cnt;
+        }
+      }
+    } else {
+      const double dx = nd.dim ? vy : vx;
+      const double an = nd.dim ? a.y : a.x;
+      const double bn = nd.dim ? b.y : b.x;
+      double dl, dr;
+      if (dx > 0.0) {
+        dl = nd.Lmax - an;
+        dr = nd.Rmin - bn;
+      } else {
+        dl = nd.Lmax - bn;
+        dr = nd.Rmin - an;
+      }
+      bool left = dl >= 0.0;
+      bool right = dr <= 0.0;
+      if (dx > 0.0) {
+        if (left) left = (dl / dx) >= 0.0;
+        if (right) right = (dr / dx) <= 1.0;
+      } else if (dx < 0.0) {
+        if (left) left = (1.0 - (dl / dx)) >= 0.0;
+        if (right) right = (1.0 - (dr / dx)) <= 1.0;
+      }
+      if (prune && (left || right)) {
+        // the two children's boxes are rows child + 1, child + 2: one aligned 64-byte pair
+        if (left) left = seg_meets_box(a, vx, vy, mseg, nbox[nd.child]);
+        if (right) right = seg_meets_box(a, vx, vy, mseg, nbox[nd.child + 1]);
+      }
+      if (left && right) {
+        stack[sp++] = nd.child;
+        node = nd.child + 1;
+        pop = false;
+      } else if (left || right) {
+        node = nd.child + (right ? 1 : 0);
+        pop = false;
+      }
+    }
+    if (pop) {
+      if (sp == 0) break;
+      node = stack[--sp];
+    }
+  }
+  return cnt;
+}
+
+// geometry_utils.py:564-574 sort_intersections_by_edge on grouped rows, np.lexsort's order
+// (stable, NaN last).  One lane per row: the row's rank inside its segment's range = rows
+// with a smaller t plus earlier rows with an equal t; the row is written once, to its rank.
+// t = (c - a) . (b - a) with c the clipped start point, x-term + y-term as the reference's
+// np.sum over the last axis.
+__device__ __forceinline__ bool t_less(double x, double y) {
+  return x < y || (!isnan(x) && isnan(y));
+}
+
+__device__ __forceinline__ void sort_row(const double* __restrict__ segs, const int64_t* __restrict__ offsets,
+                                         const int64_t* __restrict__ ii, const int64_t* __restrict__ jj_in,
+                                         const double* __restrict__ xy_in, int64_t r,
+                                         int64_t* __restrict__ jj_out, double* __restrict__ xy_out) {
+  const int64_t s = ii[r];
+  const int64_t lo = offsets[s], hi = offsets[s + 1];
+  const double4 s4 = reinterpret_cast<const double4*>(segs)[s];
+  const double abx = s4.z - s4.x, aby = s4.w - s4.y;
+  const double4 mine = reinterpret_cast<const double4*>(xy_in)[r];
+  const double t = (mine.x - s4.x) * abx + (mine.y - s4.y) * aby;
+  int64_t rank = lo;
+  for (int64_t k = lo; k < hi; ++k) {
+    if (k == r) continue;
+    const double2 c = reinterpret_cast<const double2*>(xy_in)[2 * k];
+    const double tk = (c.x - s4.x) * abx + (c.y - s4.y) * aby;
+    if (t_less(tk, t) || (k < r && !t_less(t, tk))) ++rank;
+  }
+  jj_out[rank] = jj_in[r];
+  reinterpret_cast<double4*>(xy_out)[rank] = mine;
+}
+
+}  // namespace
+}  // namespace ct

@@ -197,3 +197,30 @@ def config3(nv: int = 708, n_target_side: int = 500, seed: int = 0):
     centroids = sv[sf].mean(axis=1)
     field = sinusoid_fields(centroids, 1, 1)[:, 0]
     return sv, sf, tv, tf, field
+
+
+def config5(n_meshes: int = 512, seed: int = 0, cells=(1e3, 2e5), n_points=(1e3, 1e6),
+            n_segments=(1e2, 1e5)):
+    """Batch of independent meshes.  Mesh m: cell count log-uniform in `cells` (jittered grid,
+    jitter 0.25h, a third of the quads split: half triangles, half quads, faces width 4) on its
+    own square [x0, x0 + L] x [y0, y0 + L] (L log-uniform in [10, 1e4], origin uniform in
+    [-1e5, 1e5]^2); point count log-uniform in `n_points` (95% inside, rest in a 5% frame);
+    segment count log-uniform in `n_segments` (lengths log-uniform in [0.1h, 50h]).
+    Returns a list of (vertices, faces, points, segments)."""
+    r = _rng(seed, 60)
+    lc = np.exp(r.uniform(np.log(cells[0]), np.log(cells[1]), n_meshes))
+    npt = np.exp(r.uniform(np.log(n_points[0]), np.log(n_points[1]), n_meshes)).astype(np.int64)
+    nsg = np.exp(r.uniform(np.log(n_segments[0]), np.log(n_segments[1]), n_meshes)).astype(np.int64)
+    length = np.exp(r.uniform(np.log(10.0), np.log(1e4), n_meshes))
+    origin = r.uniform(-1e5, 1e5, (n_meshes, 2))
+    out = []
+    for m in range(n_meshes):
+        nv = max(3, int(round(np.sqrt(lc[m] * 0.75))) + 1)
+        L = float(length[m])
+        v, f = jittered_grid(nv, L, 0.25, 1.0 / 3.0, seed * 100_003 + m)
+        h = L / (nv - 1)
+        p = points(int(npt[m]), L, 0.95, 0.05 * L, seed * 100_003 + m)
+        s = segments(int(nsg[m]), L, h, 0.1, 50.0, seed * 100_003 + m)
+        o = origin[m]
+        out.append((v + o, f, p + o, s + o[None, None, :]))
+    return out

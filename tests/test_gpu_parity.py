@@ -236,3 +236,76 @@ def test_edge_tree(trees):
     oi, oj, oxy = o.intersect_edges(seg)
     assert np.array_equal(gi, oi) and np.array_equal(gj, oj)
     assert_close(gxy, oxy, L)
+
+
+def _node_boxes_host(nodes, bb_indices, bb):
+    """Tight box of every node's cells, bottom-up over the DFS numbering (children of node
+    j have larger indices), NaN-poisoned like the device kernel."""
+    n = len(nodes)
+    out = np.empty((n, 4))
+    for j in range(n - 1, -1, -1):
+        c = nodes["child"][j]
+        if c == -1:
+            b = bb[bb_indices[nodes["ptr"][j]: nodes["ptr"][j] + nodes["size"][j]]]
+            box = np.array([b[:, 0].min(), b[:, 1].max(), b[:, 2].min(), b[:, 3].max()])
+            if np.isnan(b).any():
+                box[:] = np.nan
+        else:
+            l, r = out[c], out[c + 1]
+            box = np.array([min(l[0], r[0]), max(l[1], r[1]), min(l[2], r[2]), max(l[3], r[3])])
+            if np.isnan(l).any() or np.isnan(r).any():
+                box[:] = np.nan
+        out[j] = box
+    return out
+
+
+def test_node_boxes(trees):
+    """ct_node_boxes equals the host bottom-up union of the cells' boxes, and every node's
+    box contains its children's."""
+    name, v, f, g, o = trees
+    log(f"node boxes {name}")
+    want = _node_boxes_host(o.nodes, o.bb_indices, o.bb_coords)
+    got = g.d_node_boxes.cpu().numpy()[1:]
+    assert np.array_equal(got, want)
+    assert np.isnan(g.d_node_boxes.cpu().numpy()[0]).all()
+
+
+def test_intersect_edges_pruning_is_exact(trees):
+    """Segment-face queries with and without node-box pruning give the oracle's rows, also
+    for very long, axis-parallel, degenerate and non-finite segments."""
+    name, v, f, g, o = trees
+    L = _extent(v)
+    lo = v.min(0)
+    rng = np.random.default_rng(11)
+    n = 4000
+    a = lo + rng.uniform(-0.1 * L, 1.1 * L, size=(n, 2))
+    b = lo + rng.uniform(-0.1 * L, 1.1 * L, size=(n, 2))  # long: across the mesh
+    seg = np.stack([a, b], axis=1)
+    horiz = seg[:500].copy()
+    horiz[:, 1, 1] = horiz[:, 0, 1]
+    vert = seg[500:1000].copy()
+    vert[:, 1, 0] = vert[:, 0, 0]
+    point = seg[1000:1100].copy()
+    point[:, 1] = point[:, 0]
+    # segments through the vertices and along the cell edges, end to end across the mesh
+    fv = f[:, :3]
+    edge_line = np.stack([v[fv[:300, 0]], v[fv[:300, 0]] + 50.0 * (v[fv[:300, 1]] - v[fv[:300, 0]])], axis=1)
+    bad = seg[1100:1110].copy()
+    bad[0, 0, 0] = np.nan
+    bad[1, 1, 1] = np.inf
+    bad[2, 0] = -np.inf
+    bad[3] = np.nan
+    seg = np.concatenate([seg, horiz, vert, point, edge_line, bad])
+    log(f"edges pruning {name} n={len(seg)}")
+    oi, oj, oxy = o.intersect_edges(seg)
+    gi, gj, gxy = g.intersect_edges(seg)
+    assert np.array_equal(gi, oi) and np.array_equal(gj, oj)
+    assert_close(gxy, oxy, L)
+    saved = g._ctree.node_boxes
+    try:
+        g._ctree.node_boxes = None
+        ui, uj, uxy = g.intersect_edges(seg)
+    finally:
+        g._ctree.node_boxes = saved
+    assert np.array_equal(ui, gi) and np.array_equal(uj, gj)
+    assert np.array_equal(uxy.view(np.int64), gxy.view(np.int64))
