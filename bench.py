@@ -158,6 +158,21 @@ def extras(args):
     wall, _ = time_calls(fn, 3)
     out["intersect_faces_1Mx250k_ms"] = 1e3 * wall / 3
     out["intersect_faces_pairs"] = int(fn()[0].numel())
+    if not args.no_cpu_baseline:
+        # CPU oracle on a bounded sample of the same workload: the first 50,000 of the
+        # 250,000 target cells against the whole source tree, scaled to all targets.
+        from oracle import refcpu as R
+
+        osrc = R.OracleCellTree2d(sv, sf, -1)
+        n_sample = 50_000
+        t0 = time.perf_counter()
+        osrc.intersect_faces(tv, tf[:n_sample], -1)
+        cpu_s = (time.perf_counter() - t0) * (len(tf) / n_sample)
+        out["intersect_faces_cpu_baseline"] = {
+            "value_ms": 1e3 * cpu_s, "cores": int(R.lib().ref_num_threads()), "kind": "port",
+            "sample": f"{n_sample:,} of {len(tf):,} target cells, time scaled by {len(tf) / n_sample:g}",
+        }
+        out["intersect_faces_speedup_vs_cpu"] = 1e3 * cpu_s / out["intersect_faces_1Mx250k_ms"]
     return out
 
 

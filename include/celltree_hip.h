@@ -147,9 +147,12 @@ int ct_locate_points_ordered(const ct_tree* tree, const double* points, int64_t 
                              void* stream);
 
 /* query.py:110-117 locate_points with a caller-supplied processing order (from
- * ct_query_order kind 0, or any permutation of 0..n-1); same results as ct_locate_points. */
+ * ct_query_order kind 0, or any permutation of 0..n-1); same results as ct_locate_points.
+ * `sorted` (may be NULL): the points in that order, sorted[i] = points[order[i]], as
+ * ct_query_order writes them; lane i then reads its point contiguously. */
 int ct_locate_points_in_order(const ct_tree* tree, const double* points, int64_t n,
-                              const int32_t* order, double tol, int64_t* out, void* stream);
+                              const int32_t* order, const double* sorted, double tol, int64_t* out,
+                              void* stream);
 
 /* query.py:167-174 locate_points_on_edge (EdgeCellTree2d). */
 int ct_locate_points_on_edge(const ct_tree* tree, const double* points, int64_t n, double tol,
@@ -160,30 +163,34 @@ int ct_locate_points_on_edge(const ct_tree* tree, const double* points, int64_t 
  * points by the Morton key of a 1024 x 1024 grid over `bbox` (HOST array of 4); boxes and
  * segments by a work class (log2 of their bbox area relative to `bbox`) above the Morton key
  * of their centre.  Passing the order to a count / write pass gives each wavefront
- * neighbouring queries of similar cost; results are unchanged by it. */
+ * neighbouring queries of similar cost; results are unchanged by it.  `sorted` (may be NULL)
+ * receives the queries in that order (n rows of 2 or 4 doubles). */
 size_t ct_order_workspace_bytes(int64_t n);
 int ct_query_order(int32_t kind, const double* queries, int64_t n, const double* bbox,
-                   void* workspace, size_t workspace_bytes, int32_t* order, void* stream);
+                   void* workspace, size_t workspace_bytes, int32_t* order, double* sorted,
+                   void* stream);
 
 /* query.py:278-289 locate_boxes, first pass: counts[i] = number of tree cells whose bbox
- * strictly overlaps box i.  `order` (from ct_query_order, or NULL) only schedules. */
+ * strictly overlaps box i.  `order` (from ct_query_order, or NULL) only schedules; `sorted`
+ * (NULL, or the boxes in that order from ct_query_order) lets lanes read them contiguously. */
 int ct_locate_boxes_count(const ct_tree* tree, const double* boxes, int64_t n,
-                          const int32_t* order, int64_t* counts, void* stream);
+                          const int32_t* order, const double* sorted, int64_t* counts, void* stream);
 
 /* query.py:278-289 locate_boxes, second pass: writes pairs at offsets[i] (from
  * ct_exclusive_scan), in box order then per-box DFS order (the reference's order). */
 int ct_locate_boxes_write(const ct_tree* tree, const double* boxes, int64_t n,
-                          const int32_t* order, const int64_t* offsets, int64_t* ii,
-                          int64_t* jj, void* stream);
+                          const int32_t* order, const double* sorted, const int64_t* offsets,
+                          int64_t* ii, int64_t* jj, void* stream);
 
-/* query.py:500-544 locate_edge_faces (kind 1) / locate_edge_edges (kind 0), first pass. */
+/* query.py:500-544 locate_edge_faces (kind 1) / locate_edge_edges (kind 0), first pass;
+ * `order` and `sorted` as for ct_locate_boxes_count. */
 int ct_locate_edges_count(const ct_tree* tree, const double* segments, int64_t n, int32_t kind,
-                          const int32_t* order, int64_t* counts, void* stream);
+                          const int32_t* order, const double* sorted, int64_t* counts, void* stream);
 
 /* Second pass: ii, jj and xy (n_found, 2, 2) in segment order then DFS order. */
 int ct_locate_edges_write(const ct_tree* tree, const double* segments, int64_t n, int32_t kind,
-                          const int32_t* order, const int64_t* offsets, int64_t* ii,
-                          int64_t* jj, double* xy, void* stream);
+                          const int32_t* order, const double* sorted, const int64_t* offsets,
+                          int64_t* ii, int64_t* jj, double* xy, void* stream);
 
 /* geometry_utils.py:564-574 sort_intersections_by_edge on grouped output (ii ascending, the
  * rows of segment s in [offsets[s], offsets[s+1])): writes jj and xy (n_rows, 2, 2) stably
@@ -289,14 +296,40 @@ int ct_regrid_apply_transpose(const int64_t* src_offsets, const int64_t* perm, c
                               const double* area, const double* tgt_area, const double* gy,
                               int32_t n_fields, int64_t n_src, double* gF, void* stream);
 
-/* Adds dL/d(tgt_vertices) and dL/d(src_vertices) for dL/dy = gy (caller zeroes the outputs):
- * through every pair's clip (reverse sweep over its Sutherland-Hodgman stages) and through
- * every target's area. */
-int ct_regrid_vertex_backward(const double* tgt_vertices, const int32_t* tgt_faces, int32_t n_max_t,
-                              const double* src_vertices, const int32_t* src_faces, int32_t n_max_s,
-                              const int64_t* tt, const int64_t* ss, int64_t n_pairs, int64_t n_tgt,
-                              const double* F, const double* y, const double* gy,
-                              const double* tgt_area, int32_t n_fields, double* g_tgt_vertices,
+/* The fixed pair structure of a regrid (device arrays).  Pairs are grouped by target in
+ * ascending order (intersect_faces order); corner lists map every vertex to the face corners
+ * (face * n_max + slot, ascending) that use it. */
+typedef struct ct_regrid_topology {
+  const int32_t* tgt_faces;          /* (n_tgt, n_max_t), counter-clockwise, -1 padded */
+  const int32_t* src_faces;          /* (n_src, n_max_s), counter-clockwise, -1 padded */
+  const int64_t* tt;                 /* (n_pairs) target of each pair */
+  const int64_t* ss;                 /* (n_pairs) source of each pair */
+  const int64_t* tgt_offsets;        /* (n_tgt + 1): pairs of target t */
+  const int64_t* src_offsets;        /* (n_src + 1): perm_by_src rows of source s */
+  const int64_t* perm_by_src;        /* (n_pairs) pairs ordered by source (stable) */
+  const int64_t* tgt_vertex_offsets; /* (n_tgt_vertices + 1) */
+  const int64_t* tgt_vertex_corners; /* corners of the target faces, grouped by vertex */
+  const int64_t* src_vertex_offsets; /* (n_src_vertices + 1) */
+  const int64_t* src_vertex_corners; /* corners of the source faces, grouped by vertex */
+  int64_t n_pairs;
+  int64_t n_tgt;
+  int64_t n_src;
+  int64_t n_tgt_vertices;
+  int64_t n_src_vertices;
+  int32_t n_max_t;                   /* 3 .. 8 */
+  int32_t n_max_s;                   /* 3 .. 8 */
+} ct_regrid_topology;
+
+size_t ct_regrid_backward_workspace_bytes(const ct_regrid_topology* topo);
+
+/* Writes dL/d(tgt_vertices) and dL/d(src_vertices) for dL/dy = gy: through every pair's
+ * overlap area (the derivative of the clipped polygon's area along the input edges it lies
+ * on) and through every target's area.  No atomics: per-pair corner gradients are reduced
+ * per face, then gathered per vertex, in a fixed order (run-to-run identical). */
+int ct_regrid_vertex_backward(const ct_regrid_topology* topo, const double* tgt_vertices,
+                              const double* src_vertices, const double* F, const double* y,
+                              const double* gy, const double* tgt_area, int32_t n_fields,
+                              void* workspace, size_t workspace_bytes, double* g_tgt_vertices,
                               double* g_src_vertices, void* stream);
 
 /* Adjoint of barycentric_triangle.py:28-64 with each point's face held fixed: for dL/dw
@@ -336,7 +369,7 @@ typedef struct ct_batch {
   int32_t max_depth;            /* largest tree depth */
   int32_t stack;                /* traversal stack class, >= max_depth: 16, 24, 32, 48, 64, 128, 256 */
   int32_t n_workgroups;         /* persistent grid size */
-  int32_t block;                /* workgroup size: 64, 128, 256 or 512 */
+  int32_t block;                /* workgroup size: 64, 128 or 256 */
   int32_t reserved;
 } ct_batch;
 

@@ -88,14 +88,17 @@ def exclusive_scan(counts: torch.Tensor):
     return offsets, total
 
 
-def query_order(kind: int, queries: torch.Tensor, bbox) -> torch.Tensor:
-    """Processing order (int32, device) of n queries: kind 0 points, 1 boxes, 2 segments."""
+def query_order(kind: int, queries: torch.Tensor, bbox, with_sorted: bool = False):
+    """Processing order (int32, device) of n queries: kind 0 points, 1 boxes, 2 segments.
+    With ``with_sorted``, returns (order, sorted) where sorted[i] = queries[order[i]]."""
     n = int(queries.shape[0])
     order = empty(n, torch.int32)
+    sorted_q = empty((n, 2 if kind == 0 else 4), torch.float64) if with_sorted else None
     ws = empty(int(_native.lib().ct_order_workspace_bytes(n)), torch.uint8)
     bb = (ctypes.c_double * 4)(*[float(x) for x in bbox])
-    _native.call("ct_query_order", kind, ptr(queries), n, bb, ptr(ws), ws.numel(), ptr(order), stream())
-    return order
+    _native.call("ct_query_order", kind, ptr(queries), n, bb, ptr(ws), ws.numel(), ptr(order), ptr(sorted_q),
+                 stream())
+    return (order, sorted_q) if with_sorted else order
 
 
 # ------------------------------------------------------------------ node records

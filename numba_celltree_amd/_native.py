@@ -82,6 +82,31 @@ class CtBatch(ctypes.Structure):
     ]
 
 
+class CtRegridTopology(ctypes.Structure):
+    """``ct_regrid_topology``: the fixed pair structure of a regrid (device arrays)."""
+
+    _fields_ = [
+        ("tgt_faces", ctypes.c_void_p),
+        ("src_faces", ctypes.c_void_p),
+        ("tt", ctypes.c_void_p),
+        ("ss", ctypes.c_void_p),
+        ("tgt_offsets", ctypes.c_void_p),
+        ("src_offsets", ctypes.c_void_p),
+        ("perm_by_src", ctypes.c_void_p),
+        ("tgt_vertex_offsets", ctypes.c_void_p),
+        ("tgt_vertex_corners", ctypes.c_void_p),
+        ("src_vertex_offsets", ctypes.c_void_p),
+        ("src_vertex_corners", ctypes.c_void_p),
+        ("n_pairs", ctypes.c_int64),
+        ("n_tgt", ctypes.c_int64),
+        ("n_src", ctypes.c_int64),
+        ("n_tgt_vertices", ctypes.c_int64),
+        ("n_src_vertices", ctypes.c_int64),
+        ("n_max_t", ctypes.c_int32),
+        ("n_max_s", ctypes.c_int32),
+    ]
+
+
 assert ctypes.sizeof(CtNode) == 32
 assert ctypes.sizeof(CtTree) == 120
 assert ctypes.sizeof(CtBatchTile) == 16
@@ -93,6 +118,7 @@ _F64 = ctypes.c_double
 _SZ = ctypes.c_size_t
 _TP = ctypes.POINTER(CtTree)
 _BP = ctypes.POINTER(CtBatch)
+_RP = ctypes.POINTER(CtRegridTopology)
 
 # name -> (restype, argtypes); mirrors include/celltree_hip.h one to one.
 SIGNATURES = {
@@ -116,14 +142,14 @@ SIGNATURES = {
     "ct_node_boxes": (ctypes.c_int, [_TP, _P, _P, _P]),
     "ct_locate_workspace_bytes": (_SZ, [_I64]),
     "ct_locate_points_ordered": (ctypes.c_int, [_TP, _P, _I64, _F64, _P, _SZ, _P, _P]),
-    "ct_locate_points_in_order": (ctypes.c_int, [_TP, _P, _I64, _P, _F64, _P, _P]),
+    "ct_locate_points_in_order": (ctypes.c_int, [_TP, _P, _I64, _P, _P, _F64, _P, _P]),
     "ct_locate_points_on_edge": (ctypes.c_int, [_TP, _P, _I64, _F64, _P, _P]),
     "ct_order_workspace_bytes": (_SZ, [_I64]),
-    "ct_query_order": (ctypes.c_int, [_I32, _P, _I64, ctypes.POINTER(_F64), _P, _SZ, _P, _P]),
-    "ct_locate_boxes_count": (ctypes.c_int, [_TP, _P, _I64, _P, _P, _P]),
-    "ct_locate_boxes_write": (ctypes.c_int, [_TP, _P, _I64, _P, _P, _P, _P, _P]),
-    "ct_locate_edges_count": (ctypes.c_int, [_TP, _P, _I64, _I32, _P, _P, _P]),
-    "ct_locate_edges_write": (ctypes.c_int, [_TP, _P, _I64, _I32, _P, _P, _P, _P, _P, _P]),
+    "ct_query_order": (ctypes.c_int, [_I32, _P, _I64, ctypes.POINTER(_F64), _P, _SZ, _P, _P, _P]),
+    "ct_locate_boxes_count": (ctypes.c_int, [_TP, _P, _I64, _P, _P, _P, _P]),
+    "ct_locate_boxes_write": (ctypes.c_int, [_TP, _P, _I64, _P, _P, _P, _P, _P, _P]),
+    "ct_locate_edges_count": (ctypes.c_int, [_TP, _P, _I64, _I32, _P, _P, _P, _P]),
+    "ct_locate_edges_write": (ctypes.c_int, [_TP, _P, _I64, _I32, _P, _P, _P, _P, _P, _P, _P]),
     "ct_sort_intersections_by_edge": (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _P, _P, _P]),
     "ct_scan_workspace_bytes": (_SZ, [_I64]),
     "ct_exclusive_scan": (ctypes.c_int, [_P, _I64, _P, _P, _SZ, _P]),
@@ -141,10 +167,8 @@ SIGNATURES = {
     "ct_regrid_apply": (ctypes.c_int, [_P, _P, _P, _P, _P, _I32, _I64, _P, _P]),
     "ct_regrid_apply_transpose": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I32, _I64, _P, _P]),
     "ct_barycentric_triangle_backward": (ctypes.c_int, [_P, _P, _I64, _P, _I32, _P, _P, _P, _P, _P]),
-    "ct_regrid_vertex_backward": (
-        ctypes.c_int,
-        [_P, _P, _I32, _P, _P, _I32, _P, _P, _I64, _I64, _P, _P, _P, _P, _I32, _P, _P, _P],
-    ),
+    "ct_regrid_backward_workspace_bytes": (_SZ, [_RP]),
+    "ct_regrid_vertex_backward": (ctypes.c_int, [_RP, _P, _P, _P, _P, _P, _P, _I32, _P, _SZ, _P, _P, _P]),
     "ct_batch_locate_points": (ctypes.c_int, [_BP, _P, _P, _P]),
     "ct_batch_edges_count": (ctypes.c_int, [_BP, _P, _P, _P]),
     "ct_batch_edges_write": (ctypes.c_int, [_BP, _P, _P, _I64, _P, _P, _P, _P]),

@@ -16,6 +16,7 @@ launch shape (workgroup size, workgroups per CU, tile size, stack class, queue c
 ``BatchConfig``; ``autotune`` times candidates on the actual batch and keeps the fastest.
 """
 
+import copy
 import ctypes
 import dataclasses
 import heapq
@@ -38,7 +39,7 @@ TILE_DTYPE = np.dtype([("begin", "<i8"), ("count", "<i4"), ("tree", "<i4")])
 class BatchConfig:
     """Launch shape of a batched persistent kernel."""
 
-    block: int = 256        # workgroup size (64, 128, 256, 512)
+    block: int = 256        # workgroup size (64, 128, 256)
     wg_per_cu: int = 4      # persistent workgroups per compute unit
     tile: int = 256         # queries per tile (a multiple of 64, taken by one wavefront)
     stack: int = 0          # traversal stack class; 0 = the smallest class >= max depth
@@ -167,6 +168,22 @@ class BatchPlan:
     @property
     def ref(self):
         return ctypes.byref(self._struct)
+
+    def relaunch(self, config: BatchConfig) -> "BatchPlan":
+        """The same tiles and queues under another workgroup size, grid or stack class
+        (``config.tile`` and ``config.n_queues`` must match this plan's)."""
+        if (config.tile, config.n_queues) != (self.config.tile, self.config.n_queues):
+            raise ValueError("relaunch keeps the tiles: tile and n_queues must match")
+        stack = config.stack or stack_class(self.batch.max_depth)
+        if stack not in STACK_CLASSES or stack < self.batch.max_depth:
+            raise ValueError(f"stack {stack} cannot hold depth {self.batch.max_depth}")
+        other = copy.copy(self)
+        other.config = config
+        other._struct = _native.CtBatch.from_buffer_copy(self._struct)
+        other._struct.stack = stack
+        other._struct.n_workgroups = max(1, min(65536, self.batch.n_cu * int(config.wg_per_cu)))
+        other._struct.block = int(config.block)
+        return other
 
     # ------------------------------------------------------------ kernels
     def locate_points(self, points: torch.Tensor, out: torch.Tensor) -> None:
@@ -394,8 +411,12 @@ def autotune(batch: TreeBatch, kind: str, queries: torch.Tensor, counts: Sequenc
     n = int(np.sum(counts))
     out = D.empty(n, torch.int64)
     timings = []
+    tiled: Dict[Tuple[int, int], BatchPlan] = {}  # plans differ only in tiles and queues
     for cfg in candidates:
-        plan = batch.plan(counts, cfg)
+        tk = (cfg.tile, cfg.n_queues)
+        if tk not in tiled:
+            tiled[tk] = batch.plan(counts, cfg)
+        plan = tiled[tk].relaunch(cfg)
 
         def step():
             if kind == "points":

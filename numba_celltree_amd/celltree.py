@@ -70,14 +70,18 @@ class CellTree2d(CellTree2dBase):
         out = self._locate_points_dev(pts, float(tolerance))
         return io.out(out)
 
-    def _locate_points_dev(self, pts: torch.Tensor, tolerance: float, order=None) -> torch.Tensor:
+    def _locate_points_dev(self, pts: torch.Tensor, tolerance: float, order=None, sorted_pts=None) -> torch.Tensor:
+        """``order`` / ``sorted_pts``: a processing order from ``D.query_order`` and, optionally,
+        the points in that order (then read contiguously)."""
         n = int(pts.shape[0])
         out = D.empty(n, torch.int64)
         if order is None and self.ORDERED_MIN is not None and n >= self.ORDERED_MIN:
+            # The order alone: writing the points in sorted order costs more in the sort
+            # (0.3 -> 0.8 ms at 1e7 points) than the contiguous read saves in the traversal.
             order = D.query_order(0, pts, self._bbox)
         if order is not None:
-            _native.call("ct_locate_points_in_order", self.tree_ref, D.ptr(pts), n, D.ptr(order), tolerance,
-                         D.ptr(out), D.stream())
+            _native.call("ct_locate_points_in_order", self.tree_ref, D.ptr(pts), n, D.ptr(order), D.ptr(sorted_pts),
+                         tolerance, D.ptr(out), D.stream())
         else:
             _native.call("ct_locate_points", self.tree_ref, D.ptr(pts), n, tolerance, D.ptr(out), D.stream())
         return out
@@ -85,15 +89,16 @@ class CellTree2d(CellTree2dBase):
     def _locate_boxes_dev(self, d_boxes: torch.Tensor):
         n = int(d_boxes.shape[0])
         st = D.stream()
-        order = D.query_order(1, d_boxes, self._bbox) if n >= self.ORDER_MIN else None
+        order, srt = self._order(1, d_boxes)
         counts = D.empty(n, torch.int64)
-        _native.call("ct_locate_boxes_count", self.tree_ref, D.ptr(d_boxes), n, D.ptr(order), D.ptr(counts), st)
+        _native.call("ct_locate_boxes_count", self.tree_ref, D.ptr(d_boxes), n, D.ptr(order), D.ptr(srt),
+                     D.ptr(counts), st)
         offsets, total = D.exclusive_scan(counts)
         ii = D.empty(total, torch.int64)
         jj = D.empty(total, torch.int64)
         _native.call(
-            "ct_locate_boxes_write", self.tree_ref, D.ptr(d_boxes), n, D.ptr(order), D.ptr(offsets), D.ptr(ii),
-            D.ptr(jj), st,
+            "ct_locate_boxes_write", self.tree_ref, D.ptr(d_boxes), n, D.ptr(order), D.ptr(srt), D.ptr(offsets),
+            D.ptr(ii), D.ptr(jj), st,
         )
         return ii, jj
 

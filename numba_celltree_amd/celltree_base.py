@@ -27,6 +27,16 @@ class CellTree2dBase:
     # Box and segment batches at least this large are processed in the order of
     # ct_query_order (similar work per wavefront); results do not depend on it.
     ORDER_MIN = 4096
+    # Whether ordered box / segment passes also read their queries from a sorted copy.
+    SORTED_QUERIES = False
+
+    def _order(self, kind: int, queries: torch.Tensor):
+        """(order, sorted queries or None) for a box / segment batch, or (None, None)."""
+        if int(queries.shape[0]) < self.ORDER_MIN:
+            return None, None
+        if self.SORTED_QUERIES:
+            return D.query_order(kind, queries, self._bbox, with_sorted=True)
+        return D.query_order(kind, queries, self._bbox), None
 
     # ------------------------------------------------------------------ build
     def _build_tree(self, vertices: torch.Tensor, elements: torch.Tensor, bb_coords: torch.Tensor,
@@ -198,18 +208,19 @@ class CellTree2dBase:
         """count -> scan -> write, then the per-segment sort by t (device tensors)."""
         n = int(edge_coords.shape[0])
         st = D.stream()
-        order = D.query_order(2, edge_coords, self._bbox) if n >= self.ORDER_MIN else None
+        order, srt = self._order(2, edge_coords)
         counts = D.empty(n, torch.int64)
         _native.call(
-            "ct_locate_edges_count", self.tree_ref, D.ptr(edge_coords), n, kind, D.ptr(order), D.ptr(counts), st
+            "ct_locate_edges_count", self.tree_ref, D.ptr(edge_coords), n, kind, D.ptr(order), D.ptr(srt),
+            D.ptr(counts), st
         )
         offsets, total = D.exclusive_scan(counts)
         ii = D.empty(total, torch.int64)
         jj = D.empty(total, torch.int64)
         xy = D.empty((total, 2, 2), torch.float64)
         _native.call(
-            "ct_locate_edges_write", self.tree_ref, D.ptr(edge_coords), n, kind, D.ptr(order), D.ptr(offsets),
-            D.ptr(ii), D.ptr(jj), D.ptr(xy), st,
+            "ct_locate_edges_write", self.tree_ref, D.ptr(edge_coords), n, kind, D.ptr(order), D.ptr(srt),
+            D.ptr(offsets), D.ptr(ii), D.ptr(jj), D.ptr(xy), st,
         )
         jj_sorted = torch.empty_like(jj)
         xy_sorted = torch.empty_like(xy)

@@ -25,6 +25,10 @@ namespace ct {
 namespace {
 
 constexpr int kWave = 64;
+// Largest workgroup of the persistent kernels.  Without a bound the compiler assumes 1024
+// threads (4 waves per SIMD), caps a lane at 128 VGPRs and spills the segment traversal to
+// scratch; with 256 (one wave per SIMD) it may use what the traversal needs.
+constexpr int kBatchMaxBlock = 256;
 
 struct BatchArgs {
   const ct_tree* trees;
@@ -52,7 +56,7 @@ __device__ __forceinline__ int next_tile(const BatchArgs& a, int& queue, int& mi
 }
 
 template <int MAXV, int STACK>
-__global__ void k_batch_points(BatchArgs a, const double* __restrict__ pts, int64_t* __restrict__ out) {
+__global__ void __launch_bounds__(kBatchMaxBlock) k_batch_points(BatchArgs a, const double* __restrict__ pts, int64_t* __restrict__ out) {
   int queue = blockIdx.x % a.n_queues, misses = 0;
   const int lane = threadIdx.x & (kWave - 1);
   for (int tid = next_tile(a, queue, misses); tid >= 0; tid = next_tile(a, queue, misses)) {
@@ -72,7 +76,7 @@ __global__ void k_batch_points(BatchArgs a, const double* __restrict__ pts, int6
 // Segment-face queries.  WRITE false: counts[q]; WRITE true: rows at offsets[q] when the
 // query's rows end within `capacity` (ii = concatenated query index, jj = the tree's cell).
 template <bool WRITE, int MAXV, int STACK>
-__global__ void k_batch_edges(BatchArgs a, const double* __restrict__ segs, int64_t* __restrict__ counts,
+__global__ void __launch_bounds__(kBatchMaxBlock) k_batch_edges(BatchArgs a, const double* __restrict__ segs, int64_t* __restrict__ counts,
                               const int64_t* __restrict__ offsets, int64_t capacity,
                               int64_t* __restrict__ ii, int64_t* __restrict__ jj,
                               double* __restrict__ xy) {
@@ -179,8 +183,7 @@ int check_batch(const ct_batch* b) {
              "stack must be one of 16, 24, 32, 48, 64, 128, 256");
   CT_REQUIRE(s >= b->max_depth, "stack smaller than the deepest tree");
   CT_REQUIRE(b->n_workgroups >= 1 && b->n_workgroups <= 65536, "n_workgroups must be in [1, 65536]");
-  CT_REQUIRE(b->block == 64 || b->block == 128 || b->block == 256 || b->block == 512,
-             "block must be 64, 128, 256 or 512");
+  CT_REQUIRE(b->block == 64 || b->block == 128 || b->block == 256, "block must be 64, 128 or 256");
   return CT_OK;
 }
 

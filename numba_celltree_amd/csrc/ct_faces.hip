@@ -1,6 +1,7 @@
 // ct_faces.hip -- pair-parallel FP64 polygon kernels: separating-axis test
 // (separating_axis.py) and Sutherland-Hodgman clip area (sutherland_hodgman.py), one lane
 // per (query face, tree face) candidate pair, both polygons in registers.
+#include "ct_clip.h"
 #include "ct_internal.h"
 #include "ct_primitives.h"
 
@@ -50,83 +51,6 @@ __device__ __forceinline__ bool no_separating_axis(const P2 (&a)[MA], int la, co
     }
   }
   return true;
-}
-
-// sutherland_hodgman.py:56-60 (strict)
-__device__ __forceinline__ bool sh_inside(P2 p, P2 r, double ux, double uy) {
-  return ux * (p.y - r.y) > uy * (p.x - r.x);
-}
-
-// sutherland_hodgman.py:63-74
-__device__ __forceinline__ bool sh_intersection(P2 a, double vx, double vy, P2 r, double nx, double ny,
-                                                P2* out) {
-  const double nw = dot(nx, ny, r.x - a.x, r.y - a.y);
-  const double nv = dot(nx, ny, vx, vy);
-  if (nv != 0) {
-    const double t = nw / nv;
-    *out = {a.x + t * vx, a.y + t * vy};
-    return true;
-  }
-  return false;
-}
-
-// sutherland_hodgman.py:84-148 polygon_polygon_clip_area(subject, clipper).  The working
-// polygons hold CAP = MA + MB vertices: a convex n-gon clipped by a convex m-gon has at
-// most n + m; writes past CAP are dropped.
-template <int MA, int MB>
-__device__ __forceinline__ double clip_area(const P2 (&subj)[MA], int ls, const P2 (&clip)[MB], int lc) {
-  constexpr int CAP = MA + MB;
-  P2 outp[CAP], inp[CAP];
-#pragma unroll
-  for (int i = 0; i < CAP; ++i) outp[i] = (i < MA) ? subj[i < MA ? i : 0] : P2{0.0, 0.0};
-  int n_out = ls;
-  P2 r = clip[0];
-#pragma unroll
-  for (int i = 0; i < MB; ++i)
-    if (i == lc - 1) r = clip[i];
-  for (int ci = 0; ci < lc; ++ci) {
-    P2 s = clip[0];
-#pragma unroll
-    for (int i = 0; i < MB; ++i)
-      if (i == ci) s = clip[i];
-    const double ux = s.x - r.x, uy = s.y - r.y;
-    if (ux == 0 && uy == 0) continue;
-    const double nx = -uy, ny = ux;
-    const int length = n_out;
-#pragma unroll
-    for (int j = 0; j < CAP; ++j) inp[j] = outp[j];
-    n_out = 0;
-    P2 a = inp[0];
-    for (int j = 0; j < length; ++j)
-      if (j == length - 1) a = inp[j];
-    bool a_in = sh_inside(a, r, ux, uy);
-    for (int j = 0; j < length; ++j) {
-      const P2 b = inp[j];
-      const double vx = b.x - a.x, vy = b.y - a.y;
-      if (vx == 0 && vy == 0) continue;
-      bool b_in = sh_inside(b, r, ux, uy);
-      if (b_in) {
-        if (!a_in) {
-          P2 pt;
-          if (sh_intersection(a, vx, vy, r, nx, ny, &pt) && n_out < CAP) outp[n_out++] = pt;
-        }
-        if (n_out < CAP) outp[n_out++] = b;
-      } else if (a_in) {
-        P2 pt;
-        if (sh_intersection(a, vx, vy, r, nx, ny, &pt)) {
-          if (n_out < CAP) outp[n_out++] = pt;
-        } else {
-          b_in = true;
-          if (n_out < CAP) outp[n_out++] = b;
-        }
-      }
-      a = b;
-      a_in = b_in;
-    }
-    if (n_out < 3) return 0.0;
-    r = s;
-  }
-  return polygon_area<CAP>(outp, n_out);
 }
 
 template <int MAXV>

@@ -29,13 +29,13 @@ inline unsigned grid_xcd(int64_t n) {
 // One lane per query; lane i answers query order[i] (or i).
 template <int MAXV, int STACK, bool ON_EDGE, bool PACKED>
 __global__ void __launch_bounds__(kBlock) k_locate_points(TreeView t, const double* __restrict__ pts,
-                                                          const int32_t* __restrict__ order,
-                                                          int64_t n, double tol,
+                                                          QueryOrder ord, int64_t n, double tol,
                                                           int64_t* __restrict__ out) {
   const int64_t i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int64_t q = query_index(order, i);
-  out[q] = locate_one<MAXV, STACK, ON_EDGE, PACKED>(t, ld_point(pts, q), tol);
+  const int64_t q = query_index(ord.order, i);
+  const P2 p = ord.sorted ? ld_point(ord.sorted, i) : ld_point(pts, q);
+  out[q] = locate_one<MAXV, STACK, ON_EDGE, PACKED>(t, p, tol);
 }
 
 // ----------------------------------------------------------------- query ordering
@@ -138,11 +138,22 @@ __global__ void __launch_bounds__(kBlock) k_sort_hist(int kind, const double* __
   T[(int64_t)threadIdx.x * ntiles + blockIdx.x] = hist[threadIdx.x];
 }
 
+// The query rows (2 or 4 doubles) go with their key into the coarse runs when `cq` is set,
+// so the fine pass can write the queries in final order without gathering them.
+__device__ __forceinline__ void copy_row(int kind, const double* __restrict__ from, int64_t i,
+                                         double* __restrict__ to, int64_t j) {
+  if (kind == 0) {
+    reinterpret_cast<double2*>(to)[j] = reinterpret_cast<const double2*>(from)[i];
+  } else {
+    reinterpret_cast<double4*>(to)[j] = reinterpret_cast<const double4*>(from)[i];
+  }
+}
+
 __global__ void __launch_bounds__(kBlock) k_sort_scatter(int kind, const double* __restrict__ q, int64_t n,
                                                          OrderGeom g, int64_t ntiles,
                                                          const int64_t* __restrict__ Ts,
                                                          unsigned* __restrict__ keys,
-                                                         int32_t* __restrict__ idx) {
+                                                         int32_t* __restrict__ idx, double* __restrict__ cq) {
   __shared__ int cur[kCoarse];
   cur[threadIdx.x] = (int)Ts[(int64_t)threadIdx.x * ntiles + blockIdx.x];
   __syncthreads();
@@ -158,17 +169,21 @@ __global__ void __launch_bounds__(kBlock) k_sort_scatter(int kind, const double*
     for (int j = 0; j < kSortBatch; ++j) {
       if (k[j] != ~0u) {
         const int pos = atomicAdd(&cur[k[j] >> kFineBits], 1);
+        const int64_t i = base + (int64_t)(j0 + j) * kBlock + threadIdx.x;
         keys[pos] = k[j];
-        idx[pos] = (int32_t)(base + (int64_t)(j0 + j) * kBlock + threadIdx.x);
+        idx[pos] = (int32_t)i;
+        if (cq) copy_row(kind, q, i, cq, pos);
       }
     }
   }
 }
 
-__global__ void __launch_bounds__(kFineBlock) k_sort_fine(int64_t ntiles, const int64_t* __restrict__ Ts,
+__global__ void __launch_bounds__(kFineBlock) k_sort_fine(int kind, int64_t ntiles, const int64_t* __restrict__ Ts,
                                                           int64_t n, const unsigned* __restrict__ keys,
                                                           const int32_t* __restrict__ idx,
-                                                          int32_t* __restrict__ order) {
+                                                          const double* __restrict__ cq,
+                                                          int32_t* __restrict__ order,
+                                                          double* __restrict__ sorted) {
   __shared__ int hist[kFine];
   __shared__ int lds[kFineBlock / 64];
   const int cb = blockIdx.x;
@@ -216,6 +231,7 @@ __global__ void __launch_bounds__(kFineBlock) k_sort_fine(int64_t ntiles, const 
       if (k[j] != ~0u) {
         const int pos = atomicAdd(&hist[k[j] & (kFine - 1)], 1);
         order[pos] = v[j];
+        if (sorted) copy_row(kind, cq, p0 + j * kFineBlock, sorted, pos);
       }
     }
   }
@@ -228,7 +244,9 @@ struct OrderWs {
   size_t scan_bytes;
   unsigned* keys;
   int32_t* idx;
+  double* cq;      // query rows in coarse-bin order (up to 4 doubles per query)
   int32_t* order;  // only used by ct_locate_points_ordered
+  double* sorted;  // only used by ct_locate_points_ordered
   int64_t ntiles;
 };
 
@@ -251,13 +269,16 @@ OrderWs carve_order(char* base, int64_t n, bool with_order, size_t* total) {
   w.scan_ws = take(w.scan_bytes);
   w.keys = reinterpret_cast<unsigned*>(take(sizeof(unsigned) * nn));
   w.idx = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * nn));
+  w.cq = reinterpret_cast<double*>(take(sizeof(double) * 4 * nn));
   w.order = with_order ? reinterpret_cast<int32_t*>(take(sizeof(int32_t) * nn)) : nullptr;
+  w.sorted = with_order ? reinterpret_cast<double*>(take(sizeof(double) * 2 * nn)) : nullptr;
   *total = off + 256;
   return w;
 }
 
+// `sorted` (may be null) receives the queries in the final order.
 int query_order(int kind, const double* q, int64_t n, const double* bbox, const OrderWs& w,
-                int32_t* order, hipStream_t st) {
+                int32_t* order, double* sorted, hipStream_t st) {
   OrderGeom g;
   g.bb = Box{bbox[0], bbox[1], bbox[2], bbox[3]};
   const double ex = g.bb.xmax - g.bb.xmin, ey = g.bb.ymax - g.bb.ymin;
@@ -269,9 +290,10 @@ int query_order(int kind, const double* q, int64_t n, const double* bbox, const 
   CT_LAUNCH_CHECK();
   int rc = exclusive_scan(w.T, (int64_t)kCoarse * w.ntiles, w.Ts, w.scan_ws, w.scan_bytes, st);
   if (rc) return rc;
-  k_sort_scatter<<<ntiles, kBlock, 0, st>>>(kind, q, n, g, w.ntiles, w.Ts, w.keys, w.idx);
+  double* cq = sorted ? w.cq : nullptr;
+  k_sort_scatter<<<ntiles, kBlock, 0, st>>>(kind, q, n, g, w.ntiles, w.Ts, w.keys, w.idx, cq);
   CT_LAUNCH_CHECK();
-  k_sort_fine<<<kCoarse, kFineBlock, 0, st>>>(w.ntiles, w.Ts, n, w.keys, w.idx, order);
+  k_sort_fine<<<kCoarse, kFineBlock, 0, st>>>(kind, w.ntiles, w.Ts, n, w.keys, w.idx, cq, order, sorted);
   CT_LAUNCH_CHECK();
   return CT_OK;
 }
@@ -300,14 +322,14 @@ __global__ void __launch_bounds__(kBlock) k_pack_leaf(TreeView t, double* __rest
 // pushed before the right, so the right child is visited first.
 template <bool WRITE, int STACK>
 __global__ void __launch_bounds__(kBlock) k_boxes(TreeView t, const double* __restrict__ boxes,
-                                                  const int32_t* __restrict__ order, int64_t n,
+                                                  QueryOrder ord, int64_t n,
                                                   int64_t* __restrict__ counts,
                                                   const int64_t* __restrict__ offsets,
                                                   int64_t* __restrict__ ii, int64_t* __restrict__ jj) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int64_t q = query_index(order, i);
-  const Box box = ld_box(boxes, q);
+  const int64_t q = query_index(ord.order, i);
+  const Box box = ord.sorted ? ld_box(ord.sorted, i) : ld_box(boxes, q);
   int64_t cnt = 0;
   int64_t o = WRITE ? offsets[q] : 0;
   if (boxes_intersect(box, t.bbox)) {
@@ -356,15 +378,16 @@ __global__ void __launch_bounds__(kBlock) k_boxes(TreeView t, const double* __re
 // ----------------------------------------------------------------- segments
 template <bool WRITE, int MAXV, int STACK, bool FACES>
 __global__ void __launch_bounds__(kBlock) k_edges(TreeView t, const double* __restrict__ segs,
-                                                  const int32_t* __restrict__ order, int64_t n,
+                                                  QueryOrder ord, int64_t n,
                                                   int64_t* __restrict__ counts,
                                                   const int64_t* __restrict__ offsets,
                                                   int64_t* __restrict__ ii, int64_t* __restrict__ jj,
                                                   double* __restrict__ xy) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int64_t q = query_index(order, i);
-  const double4 s4 = reinterpret_cast<const double4*>(segs)[q];
+  const int64_t q = query_index(ord.order, i);
+  const double4 s4 = ord.sorted ? reinterpret_cast<const double4*>(ord.sorted)[i]
+                                : reinterpret_cast<const double4*>(segs)[q];
   const P2 a = {s4.x, s4.y}, b = {s4.z, s4.w};
   if (WRITE) {
     edge_query<true, MAXV, STACK, FACES>(t, a, b, q, offsets[q], ii, jj, xy);
@@ -633,7 +656,7 @@ template <int MAXV, bool ON_EDGE, bool PACKED>
 struct LocateLaunch {
   const TreeView& t;
   const double* pts;
-  const int32_t* order;
+  QueryOrder order;
   int64_t n;
   double tol;
   int64_t* out;
@@ -647,13 +670,13 @@ struct LocateLaunch {
 };
 
 template <int MAXV, bool ON_EDGE, bool PACKED>
-int launch_locate_v(const TreeView& t, int depth, const double* pts, const int32_t* order, int64_t n,
+int launch_locate_v(const TreeView& t, int depth, const double* pts, QueryOrder order, int64_t n,
                     double tol, int64_t* out, hipStream_t st) {
   return with_stack(depth, LocateLaunch<MAXV, ON_EDGE, PACKED>{t, pts, order, n, tol, out, st});
 }
 
 // Face trees: picks the vertex-count class, and the packed leaf path when present.
-int launch_locate_faces(const ct_tree* tree, const double* pts, const int32_t* order, int64_t n,
+int launch_locate_faces(const ct_tree* tree, const double* pts, QueryOrder order, int64_t n,
                         double tol, int64_t* out, hipStream_t st) {
   const TreeView t = view(tree);
   const int nm = tree->n_max_vert, d = tree->depth;
@@ -673,7 +696,7 @@ template <bool WRITE>
 struct BoxesLaunch {
   const TreeView& t;
   const double* boxes;
-  const int32_t* order;
+  QueryOrder order;
   int64_t n;
   int64_t* counts;
   const int64_t* offsets;
@@ -692,7 +715,7 @@ template <bool WRITE, int MAXV, bool FACES>
 struct EdgesLaunch {
   const TreeView& t;
   const double* segs;
-  const int32_t* order;
+  QueryOrder order;
   int64_t n;
   int64_t* counts;
   const int64_t* offsets;
@@ -709,7 +732,7 @@ struct EdgesLaunch {
 };
 
 template <bool WRITE>
-int launch_edges(const ct_tree* tree, const double* segs, const int32_t* order, int64_t n, int kind,
+int launch_edges(const ct_tree* tree, const double* segs, QueryOrder order, int64_t n, int kind,
                  int64_t* counts, const int64_t* offsets, int64_t* ii, int64_t* jj, double* xy,
                  hipStream_t st) {
   const TreeView t = view(tree);
@@ -769,7 +792,7 @@ int ct_locate_points(const ct_tree* tree, const double* points, int64_t n, doubl
   if (rc) return rc;
   CT_REQUIRE(n >= 0 && tree->n_max_vert >= 3, "bad arguments");
   if (n == 0) return CT_OK;
-  return launch_locate_faces(tree, points, nullptr, n, tol, out, as_stream(stream));
+  return launch_locate_faces(tree, points, QueryOrder{nullptr, nullptr}, n, tol, out, as_stream(stream));
 }
 
 int ct_pack_leaf_coords(const ct_tree* tree, double* out, void* stream) {
@@ -788,7 +811,7 @@ size_t ct_order_workspace_bytes(int64_t n) {
 }
 
 int ct_query_order(int32_t kind, const double* queries, int64_t n, const double* bbox, void* workspace,
-                   size_t workspace_bytes, int32_t* order, void* stream) {
+                   size_t workspace_bytes, int32_t* order, double* sorted, void* stream) {
   CT_REQUIRE(kind >= 0 && kind <= 2 && n >= 0 && n < ((int64_t)1 << 31) && bbox && order,
              "bad arguments");
   if (n == 0) return CT_OK;
@@ -798,7 +821,7 @@ int ct_query_order(int32_t kind, const double* queries, int64_t n, const double*
     set_error("order workspace too small: %zu < %zu", workspace_bytes, need);
     return CT_ERR_WORKSPACE;
   }
-  return query_order(kind, queries, n, bbox, w, order, as_stream(stream));
+  return query_order(kind, queries, n, bbox, w, order, sorted, as_stream(stream));
 }
 
 size_t ct_locate_workspace_bytes(int64_t n) {
@@ -820,18 +843,20 @@ int ct_locate_points_ordered(const ct_tree* tree, const double* points, int64_t 
     return CT_ERR_WORKSPACE;
   }
   hipStream_t st = as_stream(stream);
-  rc = query_order(0, points, n, tree->bbox, w, w.order, st);
+  rc = query_order(0, points, n, tree->bbox, w, w.order, w.sorted, st);
   if (rc) return rc;
-  return launch_locate_faces(tree, points, w.order, n, tol, out, st);
+  return launch_locate_faces(tree, points, QueryOrder{w.order, w.sorted}, n, tol, out, st);
 }
 
 int ct_locate_points_in_order(const ct_tree* tree, const double* points, int64_t n,
-                              const int32_t* order, double tol, int64_t* out, void* stream) {
+                              const int32_t* order, const double* sorted, double tol, int64_t* out,
+                              void* stream) {
   int rc = check_tree(tree);
   if (rc) return rc;
   CT_REQUIRE(n >= 0 && n < ((int64_t)1 << 31) && tree->n_max_vert >= 3, "bad arguments");
+  CT_REQUIRE(sorted == nullptr || order != nullptr, "sorted queries need their order");
   if (n == 0) return CT_OK;
-  return launch_locate_faces(tree, points, order, n, tol, out, as_stream(stream));
+  return launch_locate_faces(tree, points, QueryOrder{order, sorted}, n, tol, out, as_stream(stream));
 }
 
 int ct_locate_points_on_edge(const ct_tree* tree, const double* points, int64_t n, double tol,
@@ -841,52 +866,58 @@ int ct_locate_points_on_edge(const ct_tree* tree, const double* points, int64_t 
   CT_REQUIRE(n >= 0, "n must be >= 0");
   CT_REQUIRE(tree->n_max_vert == 2, "edge tree expected (n_max_vert == 2)");
   if (n == 0) return CT_OK;
-  return launch_locate_v<4, true, false>(view(tree), tree->depth, points, nullptr, n, tol, out,
-                                         as_stream(stream));
+  return launch_locate_v<4, true, false>(view(tree), tree->depth, points, QueryOrder{nullptr, nullptr}, n, tol,
+                                         out, as_stream(stream));
 }
 
 int ct_locate_boxes_count(const ct_tree* tree, const double* boxes, int64_t n, const int32_t* order,
-                          int64_t* counts, void* stream) {
+                          const double* sorted, int64_t* counts, void* stream) {
   int rc = check_tree(tree);
   if (rc) return rc;
   CT_REQUIRE(n >= 0, "n must be >= 0");
+  CT_REQUIRE(sorted == nullptr || order != nullptr, "sorted queries need their order");
   if (n == 0) return CT_OK;
   const TreeView t = view(tree);
-  return with_stack(tree->depth, BoxesLaunch<false>{t, boxes, order, n, counts, nullptr, nullptr, nullptr,
-                                                    as_stream(stream)});
+  return with_stack(tree->depth, BoxesLaunch<false>{t, boxes, QueryOrder{order, sorted}, n, counts, nullptr,
+                                                    nullptr, nullptr, as_stream(stream)});
 }
 
 int ct_locate_boxes_write(const ct_tree* tree, const double* boxes, int64_t n, const int32_t* order,
-                          const int64_t* offsets, int64_t* ii, int64_t* jj, void* stream) {
+                          const double* sorted, const int64_t* offsets, int64_t* ii, int64_t* jj,
+                          void* stream) {
   int rc = check_tree(tree);
   if (rc) return rc;
   CT_REQUIRE(n >= 0, "n must be >= 0");
+  CT_REQUIRE(sorted == nullptr || order != nullptr, "sorted queries need their order");
   if (n == 0) return CT_OK;
   const TreeView t = view(tree);
-  return with_stack(tree->depth, BoxesLaunch<true>{t, boxes, order, n, nullptr, offsets, ii, jj,
-                                                   as_stream(stream)});
+  return with_stack(tree->depth, BoxesLaunch<true>{t, boxes, QueryOrder{order, sorted}, n, nullptr, offsets, ii,
+                                                   jj, as_stream(stream)});
 }
 
 int ct_locate_edges_count(const ct_tree* tree, const double* segments, int64_t n, int32_t kind,
-                          const int32_t* order, int64_t* counts, void* stream) {
+                          const int32_t* order, const double* sorted, int64_t* counts, void* stream) {
   int rc = check_tree(tree);
   if (rc) return rc;
   CT_REQUIRE(n >= 0 && (kind == 0 || kind == 1), "bad arguments");
   CT_REQUIRE(kind == 1 || tree->n_max_vert == 2, "edge-edge queries need an edge tree");
+  CT_REQUIRE(sorted == nullptr || order != nullptr, "sorted queries need their order");
   if (n == 0) return CT_OK;
-  return launch_edges<false>(tree, segments, order, n, kind, counts, nullptr, nullptr, nullptr, nullptr,
-                             as_stream(stream));
+  return launch_edges<false>(tree, segments, QueryOrder{order, sorted}, n, kind, counts, nullptr, nullptr,
+                             nullptr, nullptr, as_stream(stream));
 }
 
 int ct_locate_edges_write(const ct_tree* tree, const double* segments, int64_t n, int32_t kind,
-                          const int32_t* order, const int64_t* offsets, int64_t* ii, int64_t* jj,
-                          double* xy, void* stream) {
+                          const int32_t* order, const double* sorted, const int64_t* offsets, int64_t* ii,
+                          int64_t* jj, double* xy, void* stream) {
   int rc = check_tree(tree);
   if (rc) return rc;
   CT_REQUIRE(n >= 0 && (kind == 0 || kind == 1), "bad arguments");
   CT_REQUIRE(kind == 1 || tree->n_max_vert == 2, "edge-edge queries need an edge tree");
+  CT_REQUIRE(sorted == nullptr || order != nullptr, "sorted queries need their order");
   if (n == 0) return CT_OK;
-  return launch_edges<true>(tree, segments, order, n, kind, nullptr, offsets, ii, jj, xy, as_stream(stream));
+  return launch_edges<true>(tree, segments, QueryOrder{order, sorted}, n, kind, nullptr, offsets, ii, jj, xy,
+                            as_stream(stream));
 }
 
 int ct_sort_intersections_by_edge(const double* segments, const int64_t* offsets, const int64_t* ii,

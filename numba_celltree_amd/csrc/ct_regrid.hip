@@ -13,221 +13,85 @@
 // forward is a deterministic segmented sum per target.  The transpose uses a by-source
 // permutation of the pairs (also deterministic).  Vertex gradients are scattered with FP64
 // atomics (summation order may vary between runs; each contribution is exact).
+//
+// Every polygon lives in registers (ct_clip.h): loops run to the compile-time capacity with
+// a length guard and runtime-indexed accesses go through select chains, so no kernel here
+// touches scratch memory.
+#include "ct_clip.h"
 #include "ct_internal.h"
 #include "ct_primitives.h"
 
 namespace ct {
 namespace {
 
-// -------------------------------------------------------------------- forward clip
-// One Sutherland-Hodgman stage (clip edge r -> s) on polygon `in` of length `len`,
-// recording for every output vertex its provenance: src[o] = j for a copy of in[j], or
-// src[o] = -(1 + j) for the intersection of input edge (aprev[o] -> j) with the clip line.
+// arr[n] += v for a runtime n, as one predicated add per slot.
 template <int CAP>
-__device__ __forceinline__ int sh_stage(const P2 (&in)[CAP], int len, P2 r, P2 s, P2 (&out)[CAP],
-                                        int (&src)[CAP], int (&aprev)[CAP]) {
-  const double ux = s.x - r.x, uy = s.y - r.y;
-  const double nx = -uy, ny = ux;
-  int n_out = 0;
-  int ai = len - 1;
-  P2 a = in[0];
-  for (int j = 0; j < CAP; ++j)
-    if (j == len - 1) a = in[j];
-  bool a_in = ux * (a.y - r.y) > uy * (a.x - r.x);
-  for (int j = 0; j < len; ++j) {
-    const P2 b = in[j];
-    const double vx = b.x - a.x, vy = b.y - a.y;
-    if (vx == 0 && vy == 0) continue;
-    bool b_in = ux * (b.y - r.y) > uy * (b.x - r.x);
-    const double nw = nx * (r.x - a.x) + ny * (r.y - a.y);
-    const double nv = nx * vx + ny * vy;
-    if (b_in) {
-      if (!a_in && nv != 0 && n_out < CAP) {
-        const double t = nw / nv;
-        out[n_out] = {a.x + t * vx, a.y + t * vy};
-        src[n_out] = -(1 + j);
-        aprev[n_out] = ai;
-        ++n_out;
-      }
-      if (n_out < CAP) {
-        out[n_out] = b;
-        src[n_out] = j;
-        aprev[n_out] = ai;
-        ++n_out;
-      }
-    } else if (a_in) {
-      if (nv != 0) {
-        if (n_out < CAP) {
-          const double t = nw / nv;
-          out[n_out] = {a.x + t * vx, a.y + t * vy};
-          src[n_out] = -(1 + j);
-          aprev[n_out] = ai;
-          ++n_out;
-        }
-      } else {
-        b_in = true;
-        if (n_out < CAP) {
-          out[n_out] = b;
-          src[n_out] = j;
-          aprev[n_out] = ai;
-          ++n_out;
-        }
-      }
-    }
-    a = b;
-    ai = j;
-    a_in = b_in;
-  }
-  return n_out;
-}
-
-// Runs stages 0 .. upto-1 of the clip (skipping zero-length clip edges, as the reference
-// does) and returns the polygon entering stage `upto`; *dead = true if a stage left fewer
-// than 3 vertices (area 0, zero gradient).
-template <int MA, int MB, int CAP>
-__device__ __forceinline__ int sh_run(const P2 (&subj)[MA], int ls, const P2 (&clip)[MB], int lc, int upto,
-                                      P2 (&poly)[CAP], bool* dead) {
-  for (int i = 0; i < CAP; ++i) poly[i] = (i < ls) ? subj[i < MA ? i : 0] : P2{0.0, 0.0};
-  int len = ls;
-  *dead = false;
-  P2 tmp[CAP];
-  int src[CAP], aprev[CAP];
-  for (int ci = 0; ci < upto; ++ci) {
-    const P2 r = clip[(ci == 0) ? lc - 1 : ci - 1];
-    const P2 s = clip[ci];
-    if (s.x - r.x == 0 && s.y - r.y == 0) continue;
-    len = sh_stage<CAP>(poly, len, r, s, tmp, src, aprev);
-    for (int i = 0; i < CAP; ++i) poly[i] = tmp[i];
-    if (len < 3) {
-      *dead = true;
-      return len;
-    }
-  }
-  return len;
+__device__ __forceinline__ void add_at(P2 (&arr)[CAP], int n, P2 v) {
+#pragma unroll
+  for (int k = 0; k < CAP; ++k) arr[k] = sel(k == n, P2{arr[k].x + v.x, arr[k].y + v.y}, arr[k]);
 }
 
 // geometry_utils.py:82-95 polygon_area (fan from vertex 0, abs per triangle)
 template <int CAP>
 __device__ __forceinline__ double fan_area(const P2 (&p)[CAP], int len) {
-  double area = 0.0;
-  double ux = p[1].x - p[0].x, uy = p[1].y - p[0].y;
-  for (int i = 2; i < len; ++i) {
-    const double vx = p[0].x - p[i].x, vy = p[0].y - p[i].y;
-    area += fabs(ux * vy - uy * vx);
-    ux = vx;
-    uy = vy;
-  }
-  return 0.5 * area;
+  return polygon_area<CAP>(p, len);
 }
 
 // Adjoint of fan_area: g[i] += d(area)/d(p[i]) * garea.
 template <int CAP>
 __device__ __forceinline__ void fan_area_adjoint(const P2 (&p)[CAP], int len, double garea, P2 (&g)[CAP]) {
-  for (int i = 2; i < len; ++i) {
-    // term = |U x V|, U = P_{i-1} - P0 (P1 - P0 for i = 2), V = P0 - P_i
-    const int ip = i - 1;
-    double ux, uy;
-    if (i == 2) {
-      ux = p[1].x - p[0].x;
-      uy = p[1].y - p[0].y;
-    } else {
-      ux = p[0].x - p[ip].x;
-      uy = p[0].y - p[ip].y;
+#pragma unroll
+  for (int i = 2; i < CAP; ++i) {
+    if (i < len) {
+      // term = |U x V|, U = P_{i-1} - P0 (P1 - P0 for i = 2), V = P0 - P_i
+      const int ip = i - 1;
+      double ux, uy;
+      if (i == 2) {
+        ux = p[1].x - p[0].x;
+        uy = p[1].y - p[0].y;
+      } else {
+        ux = p[0].x - p[ip].x;
+        uy = p[0].y - p[ip].y;
+      }
+      const double vx = p[0].x - p[i].x, vy = p[0].y - p[i].y;
+      const double c = ux * vy - uy * vx;
+      const double gc = 0.5 * garea * ((c > 0) ? 1.0 : (c < 0 ? -1.0 : 0.0));
+      const double gux = gc * vy, guy = -gc * vx;
+      const double gvx = -gc * uy, gvy = gc * ux;
+      if (i == 2) {  // U = P1 - P0
+        g[1].x += gux;
+        g[1].y += guy;
+        g[0].x -= gux;
+        g[0].y -= guy;
+      } else {  // U = P0 - P_{i-1}
+        g[0].x += gux;
+        g[0].y += guy;
+        g[ip].x -= gux;
+        g[ip].y -= guy;
+      }
+      g[0].x += gvx;  // V = P0 - P_i
+      g[0].y += gvy;
+      g[i].x -= gvx;
+      g[i].y -= gvy;
     }
-    const double vx = p[0].x - p[i].x, vy = p[0].y - p[i].y;
-    const double c = ux * vy - uy * vx;
-    const double gc = 0.5 * garea * ((c > 0) ? 1.0 : (c < 0 ? -1.0 : 0.0));
-    const double gux = gc * vy, guy = -gc * vx;
-    const double gvx = -gc * uy, gvy = gc * ux;
-    if (i == 2) {  // U = P1 - P0
-      g[1].x += gux;
-      g[1].y += guy;
-      g[0].x -= gux;
-      g[0].y -= guy;
-    } else {  // U = P0 - P_{i-1}
-      g[0].x += gux;
-      g[0].y += guy;
-      g[ip].x -= gux;
-      g[ip].y -= guy;
-    }
-    g[0].x += gvx;  // V = P0 - P_i
-    g[0].y += gvy;
-    g[i].x -= gvx;
-    g[i].y -= gvy;
   }
 }
 
-// Adjoint of one SH stage: maps the output-vertex adjoints gout to input-vertex adjoints
-// gin and the clip-edge endpoint adjoints gr, gs.
-template <int CAP>
-__device__ __forceinline__ void sh_stage_adjoint(const P2 (&in)[CAP], int len, P2 r, P2 s, int n_out,
-                                                 const int (&src)[CAP], const int (&aprev)[CAP],
-                                                 const P2 (&gout)[CAP], P2 (&gin)[CAP], P2* gr, P2* gs) {
-  for (int i = 0; i < CAP; ++i) gin[i] = {0.0, 0.0};
-  const double ux = s.x - r.x, uy = s.y - r.y;
-  const double nx = -uy, ny = ux;
-  double gnx = 0.0, gny = 0.0;
-  for (int o = 0; o < n_out; ++o) {
-    const P2 gp = gout[o];
-    if (src[o] >= 0) {
-      gin[src[o]].x += gp.x;
-      gin[src[o]].y += gp.y;
-      continue;
-    }
-    const int j = -src[o] - 1;
-    const int ia = aprev[o];
-    const P2 a = in[ia], b = in[j];
-    const double vx = b.x - a.x, vy = b.y - a.y;
-    const double wx = r.x - a.x, wy = r.y - a.y;
-    const double nw = nx * wx + ny * wy;
-    const double nv = nx * vx + ny * vy;
-    const double t = nw / nv;
-    // p = a + t V
-    const double gt = gp.x * vx + gp.y * vy;
-    double gax = gp.x, gay = gp.y;
-    double gvx = gp.x * t, gvy = gp.y * t;
-    const double gnw = gt / nv;
-    const double gnv = -gt * t / nv;
-    // nw = N . (r - a)
-    gnx += gnw * wx;
-    gny += gnw * wy;
-    gr->x += gnw * nx;
-    gr->y += gnw * ny;
-    gax -= gnw * nx;
-    gay -= gnw * ny;
-    // nv = N . V
-    gnx += gnv * vx;
-    gny += gnv * vy;
-    gvx += gnv * nx;
-    gvy += gnv * ny;
-    // V = b - a
-    gin[j].x += gvx;
-    gin[j].y += gvy;
-    gax -= gvx;
-    gay -= gvy;
-    gin[ia].x += gax;
-    gin[ia].y += gay;
-  }
-  // N = (r.y - s.y, s.x - r.x)
-  gr->y += gnx;
-  gs->y -= gnx;
-  gs->x += gny;
-  gr->x -= gny;
-}
 
 template <int MAXV>
 __device__ __forceinline__ int load_face_idx(const int32_t* __restrict__ f, int nmax, int64_t k,
                                              const double* __restrict__ v, P2 (&poly)[MAXV], int (&vid)[MAXV]) {
   const int32_t* row = f + k * nmax;
+  int raw[MAXV];
+#pragma unroll
+  for (int i = 0; i < MAXV; ++i) raw[i] = (i < nmax) ? row[i] : kFill;
   int len = nmax < MAXV ? nmax : MAXV;
-  for (int i = 3; i < nmax && i < MAXV; ++i) {
-    if (row[i] == kFill) {
-      len = i;
-      break;
-    }
-  }
+#pragma unroll
+  for (int i = MAXV - 1; i >= 3; --i)
+    if (i < nmax && raw[i] == kFill) len = i;
+#pragma unroll
   for (int i = 0; i < MAXV; ++i) {
-    vid[i] = (i < len) ? row[i] : 0;
+    vid[i] = (i < len) ? raw[i] : 0;
     poly[i] = (i < len) ? ld_vertex(v, vid[i]) : P2{0.0, 0.0};
   }
   return len;
@@ -242,74 +106,187 @@ __global__ void __launch_bounds__(kBlock) k_pair_areas(const double* __restrict_
                                                        double* __restrict__ area) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  constexpr int CAP = MA + MB;
   P2 a[MA], b[MB];
   int ai[MA], bi[MB];
   const int la = load_face_idx<MA>(tf, ntm, tt[k], tv, a, ai);
   const int lb = load_face_idx<MB>(sf, nsm, ss[k], sv, b, bi);
-  P2 poly[CAP];
-  bool dead;
-  const int len = sh_run<MA, MB, CAP>(a, la, b, lb, lb, poly, &dead);
-  area[k] = dead ? 0.0 : fan_area<CAP>(poly, len);
+  area[k] = clip_area<MA, MB>(a, la, b, lb);
 }
 
-// Per pair: gA = sum_f gy[t,f] F[s,f] / area_t, then the reverse sweep through the clip.
+// Per pair: gA = sum_f gy[t,f] F[s,f] / area_t, times the derivative of the overlap area.
+//
+// With both polygons convex and counter-clockwise, the clip computes their intersection P,
+// and under a fixed clip path its area is a smooth function of the input vertices whose
+// derivative lives on the boundary of P: every edge of P lies on one input edge v0 -> v1
+// (d = v1 - v0), covering the parameter range [alpha, beta] of it, and moving v0 (v1) moves
+// the point at parameter u by (1 - u) (u) times the displacement.  The area changes by the
+// outward normal velocity integrated over the covered part, so with m = (d.y, -d.x)
+// (outward normal times |d|):
+//   dA/dv0 += m * ((beta - alpha) - (beta^2 - alpha^2) / 2),   dA/dv1 += m * (beta^2 - alpha^2) / 2.
+// The clip records the input edge under every edge of P (sh_run_labels), so one clip pass
+// gives the whole gradient.  It equals the derivative of the stage-by-stage arithmetic
+// (checked against finite differences and an autograd restatement of the clip).
+template <int MA, int MB, int CAP>
+__device__ __forceinline__ void overlap_area_gradient(const P2 (&a)[MA], int la, const P2 (&b)[MB], int lb,
+                                                      const P2 (&poly)[CAP], const int (&lab)[CAP], int len,
+                                                      double gA, P2 (&ga)[MA], P2 (&gb)[MB]) {
+#pragma unroll
+  for (int k = 0; k < CAP; ++k) {
+    if (k < len) {
+      const P2 p = poly[k];
+      const P2 q = (k + 1 < len) ? poly[(k + 1 < CAP) ? k + 1 : 0] : poly[0];
+      const int L = lab[k];
+      const bool on_subject = L < kClipLabel;
+      P2 v0, v1;
+      int i0, i1;
+      if (on_subject) {
+        i0 = L;
+        i1 = (L + 1 == la) ? 0 : L + 1;
+        v0 = get(a, i0);
+        v1 = get(a, i1);
+      } else {  // stage j clips against clip[j - 1] -> clip[j]
+        const int j = L - kClipLabel;
+        i0 = (j == 0) ? lb - 1 : j - 1;
+        i1 = j;
+        v0 = get(b, i0);
+        v1 = get(b, i1);
+      }
+      const double dx = v1.x - v0.x, dy = v1.y - v0.y;
+      const double dd = dx * dx + dy * dy;
+      if (dd > 0.0) {
+        const double alpha = ((p.x - v0.x) * dx + (p.y - v0.y) * dy) / dd;
+        const double beta = ((q.x - v0.x) * dx + (q.y - v0.y) * dy) / dd;
+        const double w1 = 0.5 * (beta * beta - alpha * alpha);
+        const double w0 = (beta - alpha) - w1;
+        const P2 g0 = {gA * w0 * dy, -gA * w0 * dx};
+        const P2 g1 = {gA * w1 * dy, -gA * w1 * dx};
+        if (on_subject) {
+          add_at(ga, i0, g0);
+          add_at(ga, i1, g1);
+        } else {
+          add_at(gb, i0, g0);
+          add_at(gb, i1, g1);
+        }
+      }
+    }
+  }
+}
+
+// Vertex gradients without atomics, in a fixed summation order (deterministic):
+//   k_pair_backward       pair k -> GA[k] (its target's corners), GB[k] (its source's corners)
+//   k_target_corners      target t -> TC[t] = sum of GA over t's pairs (pair order), plus the
+//                         target-area term -sum_f gy y / area_t through the fan area
+//   k_source_corners      source s -> SC[s] = sum of GB over s's pairs (by-source order)
+//   k_vertex_gather       vertex v -> sum of the corner rows of v, in corner order (CSR)
+// Corner rows have one P2 per face slot (stride n_max of that mesh).
 template <int MA, int MB>
 __global__ void __launch_bounds__(kBlock) k_pair_backward(
     const double* __restrict__ tv, const int32_t* __restrict__ tf, int ntm, const double* __restrict__ sv,
     const int32_t* __restrict__ sf, int nsm, const int64_t* __restrict__ tt, const int64_t* __restrict__ ss,
     int64_t n, const double* __restrict__ F, const double* __restrict__ gy, const double* __restrict__ tarea,
-    int nf, double* __restrict__ g_tv, double* __restrict__ g_sv) {
+    int nf, double* __restrict__ GA, double* __restrict__ GB) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const int64_t t = tt[k], s = ss[k];
   double gA = 0.0;
   for (int f = 0; f < nf; ++f) gA += gy[t * nf + f] * F[s * nf + f];
   gA /= tarea[t];
-  if (gA == 0.0) return;
   constexpr int CAP = MA + MB;
-  P2 a[MA], b[MB];
-  int ai[MA], bi[MB];
-  const int la = load_face_idx<MA>(tf, ntm, t, tv, a, ai);
-  const int lb = load_face_idx<MB>(sf, nsm, s, sv, b, bi);
-  // forward to the end for the final polygon and the area adjoint
-  P2 poly[CAP];
-  bool dead;
-  const int len = sh_run<MA, MB, CAP>(a, la, b, lb, lb, poly, &dead);
-  if (dead) return;
-  P2 g[CAP];
-  for (int i = 0; i < CAP; ++i) g[i] = {0.0, 0.0};
-  fan_area_adjoint<CAP>(poly, len, gA, g);
-  P2 gclip[MB];
-  for (int i = 0; i < MB; ++i) gclip[i] = {0.0, 0.0};
-  // reverse sweep over the clip stages (recomputing each stage's input)
-  for (int ci = lb - 1; ci >= 0; --ci) {
-    const int ri = (ci == 0) ? lb - 1 : ci - 1;
-    const P2 r = b[ri], sp = b[ci];
-    if (sp.x - r.x == 0 && sp.y - r.y == 0) continue;
-    P2 in[CAP];
-    bool d2;
-    const int lin = sh_run<MA, MB, CAP>(a, la, b, lb, ci, in, &d2);
-    P2 out[CAP];
-    int src[CAP], aprev[CAP];
-    const int nout = sh_stage<CAP>(in, lin, r, sp, out, src, aprev);
-    (void)nout;
-    P2 gin[CAP], gr = {0.0, 0.0}, gs = {0.0, 0.0};
-    sh_stage_adjoint<CAP>(in, lin, r, sp, nout, src, aprev, g, gin, &gr, &gs);
-    for (int i = 0; i < CAP; ++i) g[i] = gin[i];
-    gclip[ri].x += gr.x;
-    gclip[ri].y += gr.y;
-    gclip[ci].x += gs.x;
-    gclip[ci].y += gs.y;
+  P2 ga[MA], gb[MB];
+#pragma unroll
+  for (int i = 0; i < MA; ++i) ga[i] = {0.0, 0.0};
+#pragma unroll
+  for (int i = 0; i < MB; ++i) gb[i] = {0.0, 0.0};
+  if (gA != 0.0) {
+    P2 a[MA], b[MB];
+    int ai[MA], bi[MB];
+    const int la = load_face_idx<MA>(tf, ntm, t, tv, a, ai);
+    const int lb = load_face_idx<MB>(sf, nsm, s, sv, b, bi);
+    P2 poly[CAP];
+    int lab[CAP];
+    const int len = sh_run_labels<MA, MB, CAP>(a, la, b, lb, poly, lab);
+    if (len >= 3) overlap_area_gradient<MA, MB, CAP>(a, la, b, lb, poly, lab, len, gA, ga, gb);
   }
-  for (int i = 0; i < la; ++i) {
-    atomicAdd(g_tv + 2 * ai[i], g[i].x);
-    atomicAdd(g_tv + 2 * ai[i] + 1, g[i].y);
+  double2* ra = reinterpret_cast<double2*>(GA) + k * ntm;
+  double2* rb = reinterpret_cast<double2*>(GB) + k * nsm;
+#pragma unroll
+  for (int i = 0; i < MA; ++i)
+    if (i < ntm) ra[i] = make_double2(ga[i].x, ga[i].y);
+#pragma unroll
+  for (int i = 0; i < MB; ++i)
+    if (i < nsm) rb[i] = make_double2(gb[i].x, gb[i].y);
+}
+
+// Target corners, one lane per corner (t, i) = c / nmax, c % nmax, so the lanes of a target
+// read each pair row GA[k] together: the pairs of target t are toff[t] .. toff[t + 1]
+// (intersect_faces order).  The target-area term is the fan-area adjoint, evaluated by every
+// lane of the target and kept for its own slot.
+template <int MAXV>
+__global__ void __launch_bounds__(kBlock) k_target_corners(const double* __restrict__ v, const int32_t* __restrict__ f,
+                                                           int nmax, int64_t n, const int64_t* __restrict__ toff,
+                                                           const double* __restrict__ GA,
+                                                           const double* __restrict__ y,
+                                                           const double* __restrict__ gy,
+                                                           const double* __restrict__ tarea, int nf,
+                                                           double* __restrict__ TC) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n * nmax) return;
+  const int64_t t = c / nmax;
+  const int slot = (int)(c - t * nmax);
+  double gx = 0.0, gyv = 0.0;
+  for (int64_t k = toff[t]; k < toff[t + 1]; ++k) {
+    const double2 d = reinterpret_cast<const double2*>(GA)[k * nmax + slot];
+    gx += d.x;
+    gyv += d.y;
   }
-  for (int i = 0; i < lb; ++i) {
-    atomicAdd(g_sv + 2 * bi[i], gclip[i].x);
-    atomicAdd(g_sv + 2 * bi[i] + 1, gclip[i].y);
+  double s = 0.0;
+  for (int k = 0; k < nf; ++k) s += gy[t * nf + k] * y[t * nf + k];
+  const double garea = -s / tarea[t];
+  if (garea != 0.0) {
+    P2 p[MAXV];
+    int id[MAXV];
+    const int len = load_face_idx<MAXV>(f, nmax, t, v, p, id);
+    P2 g[MAXV];
+#pragma unroll
+    for (int i = 0; i < MAXV; ++i) g[i] = {0.0, 0.0};
+    fan_area_adjoint<MAXV>(p, len, garea, g);
+    const P2 mine = get(g, slot);
+    gx += mine.x;
+    gyv += mine.y;
   }
+  reinterpret_cast<double2*>(TC)[c] = make_double2(gx, gyv);
+}
+
+// Source corners, one lane per corner: the pairs of source s are perm[soff[s] .. soff[s + 1]).
+__global__ void __launch_bounds__(kBlock) k_source_corners(int nmax, int64_t n, const int64_t* __restrict__ soff,
+                                                           const int64_t* __restrict__ perm,
+                                                           const double* __restrict__ GB, double* __restrict__ SC) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n * nmax) return;
+  const int64_t s = c / nmax;
+  const int slot = (int)(c - s * nmax);
+  double gx = 0.0, gyv = 0.0;
+  for (int64_t m = soff[s]; m < soff[s + 1]; ++m) {
+    const double2 d = reinterpret_cast<const double2*>(GB)[perm[m] * nmax + slot];
+    gx += d.x;
+    gyv += d.y;
+  }
+  reinterpret_cast<double2*>(SC)[c] = make_double2(gx, gyv);
+}
+
+// g_v[v] = sum of corner rows corners[voff[v] .. voff[v + 1]) (row index = face * nmax + slot).
+__global__ void __launch_bounds__(kBlock) k_vertex_gather(int64_t n_vertices, const int64_t* __restrict__ voff,
+                                                          const int64_t* __restrict__ corners,
+                                                          const double* __restrict__ C, double* __restrict__ g_v) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n_vertices) return;
+  double gx = 0.0, gy = 0.0;
+  for (int64_t c = voff[v]; c < voff[v + 1]; ++c) {
+    const double2 d = reinterpret_cast<const double2*>(C)[corners[c]];
+    gx += d.x;
+    gy += d.y;
+  }
+  reinterpret_cast<double2*>(g_v)[v] = make_double2(gx, gy);
 }
 
 template <int MAXV>
@@ -323,31 +300,6 @@ __global__ void __launch_bounds__(kBlock) k_polygon_areas(const double* __restri
   out[k] = fan_area<MAXV>(p, len);
 }
 
-// d(area_t): g_area_t = -sum_f gy[t,f] y[t,f] / area_t, through the fan area.
-template <int MAXV>
-__global__ void __launch_bounds__(kBlock) k_polygon_area_backward(const double* __restrict__ v,
-                                                                  const int32_t* __restrict__ f, int nmax,
-                                                                  int64_t n, const double* __restrict__ y,
-                                                                  const double* __restrict__ gy,
-                                                                  const double* __restrict__ tarea, int nf,
-                                                                  double* __restrict__ g_v) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  double s = 0.0;
-  for (int k = 0; k < nf; ++k) s += gy[t * nf + k] * y[t * nf + k];
-  const double garea = -s / tarea[t];
-  if (garea == 0.0) return;
-  P2 p[MAXV];
-  int id[MAXV];
-  const int len = load_face_idx<MAXV>(f, nmax, t, v, p, id);
-  P2 g[MAXV];
-  for (int i = 0; i < MAXV; ++i) g[i] = {0.0, 0.0};
-  fan_area_adjoint<MAXV>(p, len, garea, g);
-  for (int i = 0; i < len; ++i) {
-    atomicAdd(g_v + 2 * id[i], g[i].x);
-    atomicAdd(g_v + 2 * id[i] + 1, g[i].y);
-  }
-}
 
 // y[t, f] = sum over the target's pairs (in pair order) of area_k F[s_k, f], / area_t.
 __global__ void __launch_bounds__(kBlock) k_apply(const int64_t* __restrict__ toff, const int64_t* __restrict__ ss,
@@ -517,27 +469,69 @@ int ct_barycentric_triangle_backward(const double* points, const int64_t* face_i
   return CT_OK;
 }
 
-int ct_regrid_vertex_backward(const double* tgt_vertices, const int32_t* tgt_faces, int32_t n_max_t,
-                              const double* src_vertices, const int32_t* src_faces, int32_t n_max_s,
-                              const int64_t* tt, const int64_t* ss, int64_t n_pairs, int64_t n_tgt,
-                              const double* F, const double* y, const double* gy, const double* tgt_area,
-                              int32_t n_fields, double* g_tgt_vertices, double* g_src_vertices, void* stream) {
-  CT_REQUIRE(n_pairs >= 0 && n_tgt >= 0 && n_fields >= 1 && n_max_t >= 3 && n_max_s >= 3, "bad sizes");
+size_t ct_regrid_backward_workspace_bytes(const ct_regrid_topology* topo) {
+  if (!topo) return 0;
+  auto a = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t p2 = 2 * sizeof(double);
+  return a(p2 * topo->n_pairs * topo->n_max_t) + a(p2 * topo->n_pairs * topo->n_max_s) +
+         a(p2 * topo->n_tgt * topo->n_max_t) + a(p2 * topo->n_src * topo->n_max_s) + 256;
+}
+
+int ct_regrid_vertex_backward(const ct_regrid_topology* topo, const double* tgt_vertices,
+                              const double* src_vertices, const double* F, const double* y, const double* gy,
+                              const double* tgt_area, int32_t n_fields, void* workspace, size_t workspace_bytes,
+                              double* g_tgt_vertices, double* g_src_vertices, void* stream) {
+  CT_REQUIRE(topo && topo->n_pairs >= 0 && topo->n_tgt >= 0 && topo->n_src >= 0 && n_fields >= 1, "bad sizes");
+  const int n_max_t = topo->n_max_t, n_max_s = topo->n_max_s;
+  CT_REQUIRE(n_max_t >= 3 && n_max_t <= 8 && n_max_s >= 3 && n_max_s <= 8,
+             "regrid supports faces of 3..8 vertices");
+  CT_REQUIRE(g_tgt_vertices && g_src_vertices, "null gradient arrays");
+  const size_t need = ct_regrid_backward_workspace_bytes(topo);
+  if (workspace_bytes < need) {
+    set_error("regrid backward workspace too small: %zu < %zu", workspace_bytes, need);
+    return CT_ERR_WORKSPACE;
+  }
+  auto a = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t p2 = 2 * sizeof(double);
+  char* w = static_cast<char*>(workspace);
+  double* GA = reinterpret_cast<double*>(w);
+  w += a(p2 * topo->n_pairs * n_max_t);
+  double* GB = reinterpret_cast<double*>(w);
+  w += a(p2 * topo->n_pairs * n_max_s);
+  double* TC = reinterpret_cast<double*>(w);
+  w += a(p2 * topo->n_tgt * n_max_t);
+  double* SC = reinterpret_cast<double*>(w);
   hipStream_t st = as_stream(stream);
+  const int64_t n_pairs = topo->n_pairs, n_tgt = topo->n_tgt, n_src = topo->n_src;
   if (n_pairs > 0) {
     const unsigned g = grid_for(n_pairs);
-    CT_PAIR_DISPATCH(k_pair_backward, n_max_t, n_max_s, tgt_vertices, tgt_faces, n_max_t, src_vertices,
-                     src_faces, n_max_s, tt, ss, n_pairs, F, gy, tgt_area, n_fields, g_tgt_vertices,
-                     g_src_vertices);
+    CT_PAIR_DISPATCH(k_pair_backward, n_max_t, n_max_s, tgt_vertices, topo->tgt_faces, n_max_t, src_vertices,
+                     topo->src_faces, n_max_s, topo->tt, topo->ss, n_pairs, F, gy, tgt_area, n_fields, GA, GB);
     CT_LAUNCH_CHECK();
   }
   if (n_tgt > 0) {
+    const unsigned g = grid_for(n_tgt * n_max_t);
     if (n_max_t <= 4)
-      k_polygon_area_backward<4><<<grid_for(n_tgt), kBlock, 0, st>>>(tgt_vertices, tgt_faces, n_max_t, n_tgt, y, gy,
-                                                                     tgt_area, n_fields, g_tgt_vertices);
+      k_target_corners<4><<<g, kBlock, 0, st>>>(tgt_vertices, topo->tgt_faces, n_max_t, n_tgt, topo->tgt_offsets, GA,
+                                                y, gy, tgt_area, n_fields, TC);
     else
-      k_polygon_area_backward<8><<<grid_for(n_tgt), kBlock, 0, st>>>(tgt_vertices, tgt_faces, n_max_t, n_tgt, y, gy,
-                                                                     tgt_area, n_fields, g_tgt_vertices);
+      k_target_corners<8><<<g, kBlock, 0, st>>>(tgt_vertices, topo->tgt_faces, n_max_t, n_tgt, topo->tgt_offsets, GA,
+                                                y, gy, tgt_area, n_fields, TC);
+    CT_LAUNCH_CHECK();
+  }
+  if (n_src > 0) {
+    k_source_corners<<<grid_for(n_src * n_max_s), kBlock, 0, st>>>(n_max_s, n_src, topo->src_offsets,
+                                                                    topo->perm_by_src, GB, SC);
+    CT_LAUNCH_CHECK();
+  }
+  if (topo->n_tgt_vertices > 0) {
+    k_vertex_gather<<<grid_for(topo->n_tgt_vertices), kBlock, 0, st>>>(
+        topo->n_tgt_vertices, topo->tgt_vertex_offsets, topo->tgt_vertex_corners, TC, g_tgt_vertices);
+    CT_LAUNCH_CHECK();
+  }
+  if (topo->n_src_vertices > 0) {
+    k_vertex_gather<<<grid_for(topo->n_src_vertices), kBlock, 0, st>>>(
+        topo->n_src_vertices, topo->src_vertex_offsets, topo->src_vertex_corners, SC, g_src_vertices);
     CT_LAUNCH_CHECK();
   }
   return CT_OK;

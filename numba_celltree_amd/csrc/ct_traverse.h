@@ -68,6 +68,14 @@ __device__ __forceinline__ int64_t query_index(const int32_t* __restrict__ order
   return order ? (int64_t)order[i] : i;
 }
 
+// Processing order of a query batch: lane i answers query order[i] (i when `order` is null).
+// `sorted`, when not null, holds the queries already in that order (sorted[i] =
+// queries[order[i]]), so lane i reads its query contiguously instead of gathering it.
+struct QueryOrder {
+  const int32_t* order;
+  const double* sorted;
+};
+
 // Row k of the packed leaf coordinates: the cell's vertices, unused slots NaN.  The length
 // rule is polygon_length's (geometry_utils.py:72-79): the first empty slot at index >= 3.
 template <int MAXV>

@@ -12,6 +12,7 @@ ct_regrid_pair_areas / ct_regrid_apply / ct_regrid_apply_transpose /
 ct_regrid_vertex_backward).
 """
 
+import ctypes
 from typing import Optional
 
 import torch
@@ -26,6 +27,17 @@ def _offsets(index: torch.Tensor, n: int) -> torch.Tensor:
         n, dtype=torch.int64, device=index.device)
     offsets, _ = D.exclusive_scan(counts)
     return offsets
+
+
+def _vertex_corners(faces: torch.Tensor, n_vertices: int):
+    """(offsets (n_vertices + 1), corners): the corners face * n_max + slot of every vertex,
+    ascending (a stable sort of the corners by vertex)."""
+    flat = faces.reshape(-1).to(torch.int64)
+    ids = torch.arange(flat.numel(), dtype=torch.int64, device=flat.device)
+    valid = flat >= 0
+    v = flat[valid]
+    order = torch.argsort(v, stable=True)
+    return _offsets(v, n_vertices), ids[valid][order].contiguous()
 
 
 class RegridTopology:
@@ -51,6 +63,18 @@ class RegridTopology:
         self.tgt_offsets = _offsets(tt, self.n_tgt)
         self.perm_by_src = torch.argsort(ss, stable=True)
         self.src_offsets = _offsets(ss, self.n_src)
+        self.n_tgt_vertices = int(d_tv.shape[0])
+        self.n_src_vertices = int(tree.d_vertices.shape[0])
+        self.tgt_vertex_offsets, self.tgt_vertex_corners = _vertex_corners(d_tf, self.n_tgt_vertices)
+        self.src_vertex_offsets, self.src_vertex_corners = _vertex_corners(self.src_faces, self.n_src_vertices)
+        s = _native.CtRegridTopology()
+        for name in ("tgt_faces", "src_faces", "tt", "ss", "tgt_offsets", "src_offsets", "perm_by_src",
+                     "tgt_vertex_offsets", "tgt_vertex_corners", "src_vertex_offsets", "src_vertex_corners"):
+            setattr(s, name, getattr(self, name).data_ptr())
+        s.n_pairs, s.n_tgt, s.n_src = self.n_pairs, self.n_tgt, self.n_src
+        s.n_tgt_vertices, s.n_src_vertices = self.n_tgt_vertices, self.n_src_vertices
+        s.n_max_t, s.n_max_s = int(d_tf.shape[1]), int(self.src_faces.shape[1])
+        self._struct = s
 
     # ------------------------------------------------------------------ kernels
     def pair_areas(self, src_v: torch.Tensor, tgt_v: torch.Tensor) -> torch.Tensor:
@@ -83,13 +107,15 @@ class RegridTopology:
         return gF
 
     def vertex_backward(self, src_v, tgt_v, F, y, gy, tarea):
-        g_tv = torch.zeros_like(tgt_v)
-        g_sv = torch.zeros_like(src_v)
+        if int(tgt_v.shape[0]) != self.n_tgt_vertices or int(src_v.shape[0]) != self.n_src_vertices:
+            raise ValueError("vertex arrays must keep the sizes of the topology's meshes")
+        g_tv = torch.empty_like(tgt_v)
+        g_sv = torch.empty_like(src_v)
+        ref = ctypes.byref(self._struct)
+        ws = D.empty(int(_native.lib().ct_regrid_backward_workspace_bytes(ref)), torch.uint8)
         _native.call(
-            "ct_regrid_vertex_backward", D.ptr(tgt_v), D.ptr(self.tgt_faces), int(self.tgt_faces.shape[1]),
-            D.ptr(src_v), D.ptr(self.src_faces), int(self.src_faces.shape[1]), D.ptr(self.tt), D.ptr(self.ss),
-            self.n_pairs, self.n_tgt, D.ptr(F), D.ptr(y), D.ptr(gy), D.ptr(tarea), int(F.shape[1]),
-            D.ptr(g_tv), D.ptr(g_sv), D.stream(),
+            "ct_regrid_vertex_backward", ref, D.ptr(tgt_v), D.ptr(src_v), D.ptr(F), D.ptr(y), D.ptr(gy),
+            D.ptr(tarea), int(F.shape[1]), D.ptr(ws), ws.numel(), D.ptr(g_tv), D.ptr(g_sv), D.stream(),
         )
         return g_sv, g_tv
 

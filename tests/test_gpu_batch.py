@@ -39,7 +39,7 @@ def meshes():
 CONFIGS = [
     dict(),
     dict(block=64, wg_per_cu=2, tile=64, n_queues=1),
-    dict(block=512, wg_per_cu=1, tile=1024, n_queues=3),
+    dict(block=256, wg_per_cu=1, tile=1024, n_queues=3),
     dict(block=128, wg_per_cu=8, tile=192, n_queues=8, stack=128),
 ]
 

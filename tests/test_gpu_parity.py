@@ -128,6 +128,14 @@ def test_locate_points_every_path(trees):
                 D.ptr(out2), D.stream(),
             )
             assert np.array_equal(out2.cpu().numpy(), want), f"ordered packed={packed}"
+            for with_sorted in (False, True):
+                order, srt = D.query_order(0, d_pts, g._bbox, with_sorted=True)
+                out3 = D.empty(len(pts), torch.int64)
+                _native.call(
+                    "ct_locate_points_in_order", g.tree_ref, D.ptr(d_pts), len(pts), D.ptr(order),
+                    D.ptr(srt if with_sorted else None), tol, D.ptr(out3), D.stream(),
+                )
+                assert np.array_equal(out3.cpu().numpy(), want), f"in_order sorted={with_sorted} packed={packed}"
     finally:
         g._ctree.leaf_coords = saved
 
@@ -152,9 +160,16 @@ def test_query_order_is_permutation(kind):
         q[7] = np.inf
     else:
         q = S.segments(n, 100.0, 1.0, 0.01, 100.0, 3)
-    order = D.query_order(kind, torch.from_numpy(np.ascontiguousarray(q)).cuda(), [0.0, 100.0, 0.0, 100.0])
+    d_q = torch.from_numpy(np.ascontiguousarray(q)).cuda()
+    order = D.query_order(kind, d_q, [0.0, 100.0, 0.0, 100.0])
     o = order.cpu().numpy()
     assert np.array_equal(np.sort(o), np.arange(n))
+    # the same order, plus the queries written in it
+    order2, srt = D.query_order(kind, d_q, [0.0, 100.0, 0.0, 100.0], with_sorted=True)
+    o2 = order2.cpu().numpy()
+    assert np.array_equal(np.sort(o2), np.arange(n))
+    rows = q.reshape(n, -1)[o2]
+    assert np.array_equal(srt.cpu().numpy().view(np.int64), rows.view(np.int64))
 
 
 def test_locate_boxes_ordered(trees):

@@ -115,8 +115,9 @@ def test_torch_gradcheck_small():
     F_t = torch.from_numpy(F).cuda().requires_grad_(True)
     sv_t = torch.from_numpy(sv).cuda().requires_grad_(True)
     tv_t = rg.topology.tgt_vertices.clone().requires_grad_(True)
+    # vertex gradients are summed with FP64 atomics: the order (last bits) may vary per call
     assert torch.autograd.gradcheck(lambda a, b, c: rg(a, b, c), (F_t, sv_t, tv_t), eps=1e-6, atol=1e-9,
-                                    rtol=1e-7)
+                                    rtol=1e-7, nondet_tol=1e-12)
 
 
 # ------------------------------------------------------------- torch restatement
@@ -181,7 +182,9 @@ def _fan(poly):
 def test_matches_torch_autograd_restatement():
     import torch
 
-    sv, sf, tv, tf, F = problem(9, 5, 2.0, seed=5, n_fields=3)
+    # inset: no target edge collinear with a source boundary edge, where the clip's
+    # arithmetic has degenerate (near-parallel) intersections and no unique derivative
+    sv, sf, tv, tf, F = problem(9, 5, 2.0, seed=5, n_fields=3, inset=0.0613)
     tree, rg = build(sv, sf, tv, tf)
     topo = rg.topology
     G = np.random.default_rng(2).normal(size=(len(tf), F.shape[1]))
@@ -228,6 +231,11 @@ def test_barycentric_adjoint():
     face, w = barycentric_weights(tree, pts)
     ref_face, ref_w = tree.compute_barycentric_weights(pts)
     assert torch.equal(face, ref_face) and torch.equal(w, ref_w)
+    # The weights are |cross| / area ratios: |.| has a kink where a point crosses an edge
+    # line.  The finite-difference check keeps points at least 1e-3 (in weight) inside.
+    inside = (face >= 0) & (w.min(dim=1).values > 1e-3)
+    pts = pts[inside].contiguous()
+    face = face[inside].contiguous()
     G = torch.from_numpy(rng.normal(size=(len(pts), 3))).cuda()
     p = pts.clone().requires_grad_(True)
     vv = tree.d_vertices.clone().requires_grad_(True)
@@ -242,7 +250,7 @@ def test_barycentric_adjoint():
             return float((BarycentricTriangleFunction.apply(px, vx, tree.d_elements, face, tol) * G).sum())
 
     # fourth-order central differences: truncation O(h^4), rounding ~ eps * |loss| / h
-    h = 1e-4
+    h = 1e-6
     for name, base, grad, k in (("points", pts, p.grad, 0), ("vertices", tree.d_vertices, vv.grad, 1)):
         d = torch.from_numpy(rng.normal(size=tuple(base.shape))).cuda()
 
