@@ -336,23 +336,12 @@ __global__ void __launch_bounds__(kBlock) k_boxes(TreeView t, const double* __re
     int stack[STACK];
     int sp = 0;
     int node = 0;
+    // Internal nodes in an inner loop; the leaf's cells once the whole wavefront is at a
+    // leaf or finished (the loop shape of locate_one).
     while (true) {
-      const NodeLd nd = ld_node(t.nodes, node);
-      bool pop = true;
-      if (nd.child == -1) {
-        for (int k = nd.ptr; k < nd.ptr + nd.size; ++k) {
-          const int c = t.bb_indices[k];
-          if (boxes_intersect(box, ld_box(t.bb, c))) {
-            if (WRITE) {
-              ii[o] = q;
-              jj[o] = c;
-              ++o;
-            } else {
-              ++cnt;
-            }
-          }
-        }
-      } else {
+      NodeLd nd = ld_node(t.nodes, node);
+      bool exhausted = false;
+      while (nd.child != -1) {
         const double bmin = nd.dim ? box.ymin : box.xmin;
         const double bmax = nd.dim ? box.ymax : box.xmax;
         const bool left = bmin <= nd.Lmax;
@@ -360,16 +349,32 @@ __global__ void __launch_bounds__(kBlock) k_boxes(TreeView t, const double* __re
         if (left && right) {
           stack[sp++] = nd.child;
           node = nd.child + 1;
-          pop = false;
         } else if (left || right) {
           node = nd.child + (right ? 1 : 0);
-          pop = false;
+        } else {
+          if (sp == 0) {
+            exhausted = true;
+            break;
+          }
+          node = stack[--sp];
+        }
+        nd = ld_node(t.nodes, node);
+      }
+      if (exhausted) break;
+      for (int k = nd.ptr; k < nd.ptr + nd.size; ++k) {
+        const int c = t.bb_indices[k];
+        if (boxes_intersect(box, ld_box(t.bb, c))) {
+          if (WRITE) {
+            ii[o] = q;
+            jj[o] = c;
+            ++o;
+          } else {
+            ++cnt;
+          }
         }
       }
-      if (pop) {
-        if (sp == 0) break;
-        node = stack[--sp];
-      }
+      if (sp == 0) break;
+      node = stack[--sp];
     }
   }
   if (!WRITE) counts[q] = cnt;

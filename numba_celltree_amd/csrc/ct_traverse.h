@@ -100,51 +100,79 @@ __device__ __forceinline__ int load_leaf_poly(const double* __restrict__ leaf, i
 
 // ----------------------------------------------------------------- points
 // query.py:62-107 locate_point (faces) / :120-164 locate_point_on_edge (edges).
+//
+// Loop shape: an inner loop walks internal nodes until the lane stands on a leaf (or its
+// stack runs dry), and only then tests that leaf's cells.  Lanes of a wavefront reach
+// leaves at different iterations; with one loop per node the costly cell test would run
+// (masked) once for every iteration in which any lane sits on a leaf.  Here lanes that
+// reach a leaf early wait in the inner loop's exit, so the wavefront runs the cell test
+// for all of them together.  Each lane still visits its nodes in the reference's order.
+template <int MAXV, int STACK, bool ON_EDGE, bool PACKED>
+__device__ __forceinline__ bool leaf_hit(const TreeView& t, P2 p, double tol, const NodeLd& nd, int* found) {
+  for (int k = nd.ptr; k < nd.ptr + nd.size; ++k) {
+    if constexpr (ON_EDGE) {
+      const int c = t.bb_indices[k];
+      const int2 e = reinterpret_cast<const int2*>(t.elements)[c];
+      if (point_on_edge(p, ld_vertex(t.vertices, e.x), ld_vertex(t.vertices, e.y), tol)) {
+        *found = c;
+        return true;
+      }
+    } else if constexpr (PACKED) {
+      P2 poly[MAXV];
+      const int len = load_leaf_poly<MAXV>(t.leaf, t.nmax, k, poly);
+      if (point_in_poly_or_edge<MAXV>(p, poly, len, tol)) {
+        *found = t.bb_indices[k];
+        return true;
+      }
+    } else {
+      const int c = t.bb_indices[k];
+      P2 poly[MAXV];
+      const int len = load_poly<MAXV>(t.elements, t.nmax, c, t.vertices, poly);
+      if (point_in_poly_or_edge<MAXV>(p, poly, len, tol)) {
+        *found = c;
+        return true;
+      }
+    }
+  }
+  return false;
+}
+
 template <int MAXV, int STACK, bool ON_EDGE, bool PACKED>
 __device__ __forceinline__ int locate_one(const TreeView& t, P2 p, double tol) {
   int stack[STACK];
   int sp = 0;
   int node = 0;
   while (true) {
-    const NodeLd nd = ld_node(t.nodes, node);
-    if (nd.child == -1) {
-      for (int k = nd.ptr; k < nd.ptr + nd.size; ++k) {
-        if constexpr (ON_EDGE) {
-          const int c = t.bb_indices[k];
-          const int2 e = reinterpret_cast<const int2*>(t.elements)[c];
-          if (point_on_edge(p, ld_vertex(t.vertices, e.x), ld_vertex(t.vertices, e.y), tol)) return c;
-        } else if constexpr (PACKED) {
-          P2 poly[MAXV];
-          const int len = load_leaf_poly<MAXV>(t.leaf, t.nmax, k, poly);
-          if (point_in_poly_or_edge<MAXV>(p, poly, len, tol)) return t.bb_indices[k];
-        } else {
-          const int c = t.bb_indices[k];
-          P2 poly[MAXV];
-          const int len = load_poly<MAXV>(t.elements, t.nmax, c, t.vertices, poly);
-          if (point_in_poly_or_edge<MAXV>(p, poly, len, tol)) return c;
+    NodeLd nd = ld_node(t.nodes, node);
+    bool exhausted = false;
+    while (nd.child != -1) {
+      const double v = nd.dim ? p.y : p.x;
+      const bool left = v <= nd.Lmax;
+      const bool right = v >= nd.Rmin;
+      const int lc = nd.child, rc = lc + 1;
+      if (left && right) {
+        // Both viable: the side with the larger margin goes first (query.py:93-101).
+        const bool right_first = (nd.Lmax - v) < (v - nd.Rmin);
+        stack[sp++] = right_first ? lc : rc;
+        node = right_first ? rc : lc;
+      } else if (left) {
+        node = lc;
+      } else if (right) {
+        node = rc;
+      } else {
+        if (sp == 0) {
+          exhausted = true;
+          break;
         }
+        node = stack[--sp];
       }
-      if (sp == 0) return -1;
-      node = stack[--sp];
-      continue;
+      nd = ld_node(t.nodes, node);
     }
-    const double v = nd.dim ? p.y : p.x;
-    const bool left = v <= nd.Lmax;
-    const bool right = v >= nd.Rmin;
-    const int lc = nd.child, rc = lc + 1;
-    if (left && right) {
-      // Both viable: the side with the larger margin goes first (query.py:93-101).
-      const bool right_first = (nd.Lmax - v) < (v - nd.Rmin);
-      stack[sp++] = right_first ? lc : rc;
-      node = right_first ? rc : lc;
-    } else if (left) {
-      node = lc;
-    } else if (right) {
-      node = rc;
-    } else {
-      if (sp == 0) return -1;
-      node = stack[--sp];
-    }
+    if (exhausted) return -1;
+    int found;
+    if (leaf_hit<MAXV, STACK, ON_EDGE, PACKED>(t, p, tol, nd, &found)) return found;
+    if (sp == 0) return -1;
+    node = stack[--sp];
   }
 }
 
@@ -227,27 +255,15 @@ __device__ __forceinline__ int64_t edge_query(const TreeView& t, P2 a, P2 b, int
   // Non-finite segments keep the reference traversal unchanged.
   const bool prune = FACES && t.nbox != nullptr && isfinite(mseg) && isfinite(vx) && isfinite(vy);
   const double4* __restrict__ nbox = prune ? reinterpret_cast<const double4*>(t.nbox) + 1 : nullptr;
+  // Same loop shape as locate_one: internal nodes in an inner loop, the leaf's cell tests
+  // once every lane of the wavefront stands on a leaf or has finished.
   int stack[STACK];
   int sp = 0;
   int node = 0;
   while (true) {
-    const NodeLd nd = ld_node(t.nodes, node);
-    bool pop = true;
-    if (nd.child == -1) {
-      for (int k = nd.ptr; k < nd.ptr + nd.size; ++k) {
-        const int c = t.bb_indices[k];
-        P2 cc, dd;
-        const bool hit = FACES ? edge_face<MAXV>(t, c, a, b, &cc, &dd) : edge_edge(t, c, a, b, &cc, &dd);
-        if (hit) {
-          if (WRITE) {
-            ii[o + cnt] = q;
-            jj[o + cnt] = c;
-            reinterpret_cast<double4*>(xy)[o + cnt] = make_double4(cc.x, cc.y, dd.x, dd.y);
-          }
-          ++cnt;
-        }
-      }
-    } else {
+    NodeLd nd = ld_node(t.nodes, node);
+    bool exhausted = false;
+    while (nd.child != -1) {
       const double dx = nd.dim ? vy : vx;
       const double an = nd.dim ? a.y : a.x;
       const double bn = nd.dim ? b.y : b.x;
@@ -276,16 +292,33 @@ __device__ __forceinline__ int64_t edge_query(const TreeView& t, P2 a, P2 b, int
       if (left && right) {
         stack[sp++] = nd.child;
         node = nd.child + 1;
-        pop = false;
       } else if (left || right) {
         node = nd.child + (right ? 1 : 0);
-        pop = false;
+      } else {
+        if (sp == 0) {
+          exhausted = true;
+          break;
+        }
+        node = stack[--sp];
+      }
+      nd = ld_node(t.nodes, node);
+    }
+    if (exhausted) break;
+    for (int k = nd.ptr; k < nd.ptr + nd.size; ++k) {
+      const int c = t.bb_indices[k];
+      P2 cc, dd;
+      const bool hit = FACES ? edge_face<MAXV>(t, c, a, b, &cc, &dd) : edge_edge(t, c, a, b, &cc, &dd);
+      if (hit) {
+        if (WRITE) {
+          ii[o + cnt] = q;
+          jj[o + cnt] = c;
+          reinterpret_cast<double4*>(xy)[o + cnt] = make_double4(cc.x, cc.y, dd.x, dd.y);
+        }
+        ++cnt;
       }
     }
-    if (pop) {
-      if (sp == 0) break;
-      node = stack[--sp];
-    }
+    if (sp == 0) break;
+    node = stack[--sp];
   }
   return cnt;
 }
