@@ -59,8 +59,8 @@ typedef struct ct_tree {
   int32_t depth;             /* longest root-to-leaf path, in edges */
   double bbox[4];            /* host values: tree bounding box */
   const double* leaf_coords; /* optional (may be NULL): (n_elements, n_max_vert, 2), row i =
-                                the vertices of cell bb_indices[i], unused slots NaN; see
-                                ct_pack_leaf_coords */
+                                the vertices of cell bb_indices[i], unused slots repeating
+                                its last vertex; see ct_pack_leaf_coords */
   const double* node_boxes;  /* optional (may be NULL): (n_nodes + 1, 4), row j + 1 = the
                                 tight box of node j's cells; see ct_node_boxes */
 } ct_tree;
@@ -124,9 +124,11 @@ int ct_locate_points(const ct_tree* tree, const double* points, int64_t n, doubl
                      int64_t* out, void* stream);
 
 /* Layout helper, no reference counterpart: fills `out` (n_elements * n_max_vert * 2 doubles)
- * with the vertices of every cell in bb_indices order (unused slots NaN), so traversals read
- * a leaf's cells contiguously instead of through bb_indices -> faces -> vertices.  Requires
- * 3 <= n_max_vert <= 8 and non-NaN vertex coordinates. */
+ * with the vertices of every cell in bb_indices order, slots past the cell's polygon
+ * (polygon_length's rule) repeating its last vertex, so traversals read a leaf's cells
+ * contiguously instead of through bb_indices -> faces -> vertices, and test every cell as
+ * a fixed n_max_vert-edge loop (a repeated vertex adds a zero-length edge, which changes no
+ * point-in-polygon answer).  Requires 3 <= n_max_vert <= 8. */
 int ct_pack_leaf_coords(const ct_tree* tree, double* out, void* stream);
 
 /* Layout helper, no reference counterpart: out (n_nodes + 1, 4) gets, at row j + 1, the

@@ -57,6 +57,8 @@ __device__ __forceinline__ int next_tile(const BatchArgs& a, int& queue, int& mi
 
 template <int MAXV, int STACK>
 __global__ void __launch_bounds__(kBatchMaxBlock) k_batch_points(BatchArgs a, const double* __restrict__ pts, int64_t* __restrict__ out) {
+  __shared__ int lds_stack[LaneStack<STACK>::kLdsInts * kBatchMaxBlock];
+  LaneStack<STACK> stack(lds_stack + threadIdx.x, kBatchMaxBlock);
   int queue = blockIdx.x % a.n_queues, misses = 0;
   const int lane = threadIdx.x & (kWave - 1);
   for (int tid = next_tile(a, queue, misses); tid >= 0; tid = next_tile(a, queue, misses)) {
@@ -67,8 +69,8 @@ __global__ void __launch_bounds__(kBatchMaxBlock) k_batch_points(BatchArgs a, co
     for (int j = lane; j < tile.count; j += kWave) {
       const int64_t q = base + j;
       const P2 p = ld_point(pts, q);
-      out[q] = t.leaf ? locate_one<MAXV, STACK, false, true>(t, p, tol)
-                      : locate_one<MAXV, STACK, false, false>(t, p, tol);
+      out[q] = t.leaf ? locate_one<MAXV, STACK, false, kLeafPacked>(t, p, tol, stack)
+                      : locate_one<MAXV, STACK, false, kLeafFaces>(t, p, tol, stack);
     }
   }
 }

@@ -136,6 +136,40 @@ __device__ __forceinline__ bool point_in_poly_or_edge(P2 p, const P2 (&poly)[MAX
   return c;
 }
 
+// point_in_poly_or_edge on a row of exactly MAXV vertices whose unused tail repeats the last
+// vertex (the packed leaf rows).  Same answer as the reference's loop, with no branches on
+// the common path:
+//  - a repeated vertex (v1 == v0) gives A = fl(ux*uy) - fl(uy*ux) = 0 and W = 0, so the
+//    on-edge test fails, and v0.y == v1.y, so the edge does not straddle p.y; moving v0 to
+//    an equal point changes nothing (signed zeros compare equal), so the reference's skip
+//    is not needed, and the padded tail edges contribute nothing either;
+//  - the reference returns True at the first on-edge hit and the parity otherwise, which
+//    is (any on-edge) || parity;
+//  - in_bounds only matters once within_perp holds (rare), so it sits behind that branch;
+//  - the crossing abscissa is computed as the reference writes it, with a correctly
+//    rounded division, and only used where the edge straddles p.y.
+template <int MAXV>
+__device__ __forceinline__ bool point_in_cell(P2 p, const P2 (&poly)[MAXV], double tol) {
+  P2 v0 = poly[MAXV - 1];
+  double ux = v0.x - p.x, uy = v0.y - p.y;
+  bool c = false, on = false;
+#pragma unroll
+  for (int i = 0; i < MAXV; ++i) {
+    const P2 v1 = poly[i];
+    const double vx = v1.x - p.x, vy = v1.y - p.y;
+    const double A = cross(ux, uy, vx, vy);
+    const double wx = v1.x - v0.x, wy = v1.y - v0.y;
+    if (within_perp(A, wx, wy, tol)) on = on || in_bounds(p, v0, v1);
+    const bool straddle = (v0.y > p.y) != (v1.y > p.y);
+    const double xi = wx * (p.y - v0.y) / wy + v0.x;
+    c = c != (straddle && (p.x < xi));
+    v0 = v1;
+    ux = vx;
+    uy = vy;
+  }
+  return on || c;
+}
+
 // geometry_utils.py:226-238
 __device__ __forceinline__ bool point_on_edge(P2 p, P2 v0, P2 v1, double tol) {
   if (v1.x == v0.x && v1.y == v0.y) return false;

@@ -27,15 +27,17 @@ inline unsigned grid_xcd(int64_t n) {
 }
 
 // One lane per query; lane i answers query order[i] (or i).
-template <int MAXV, int STACK, bool ON_EDGE, bool PACKED>
+template <int MAXV, int STACK, bool ON_EDGE, int LEAF>
 __global__ void __launch_bounds__(kBlock) k_locate_points(TreeView t, const double* __restrict__ pts,
                                                           QueryOrder ord, int64_t n, double tol,
                                                           int64_t* __restrict__ out) {
+  __shared__ int lds_stack[LaneStack<STACK>::kLdsInts * kBlock];
   const int64_t i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int64_t q = query_index(ord.order, i);
   const P2 p = ord.sorted ? ld_point(ord.sorted, i) : ld_point(pts, q);
-  out[q] = locate_one<MAXV, STACK, ON_EDGE, PACKED>(t, p, tol);
+  LaneStack<STACK> stack(lds_stack + threadIdx.x, kBlock);
+  out[q] = locate_one<MAXV, STACK, ON_EDGE, LEAF>(t, p, tol, stack);
 }
 
 // ----------------------------------------------------------------- query ordering
@@ -306,14 +308,10 @@ __global__ void __launch_bounds__(kBlock) k_pack_leaf(TreeView t, double* __rest
   const int32_t* row = t.elements + (int64_t)c * t.nmax;
   const int len = poly_len(row, t.nmax);
   double2* dst = reinterpret_cast<double2*>(out) + k * t.nmax;
-  const double nan = __builtin_nan("");
+  P2 v = {0.0, 0.0};
   for (int i = 0; i < t.nmax; ++i) {
-    if (i < len) {
-      const P2 v = ld_vertex(t.vertices, row[i]);
-      dst[i] = make_double2(v.x, v.y);
-    } else {
-      dst[i] = make_double2(nan, nan);
-    }
+    if (i < len) v = ld_vertex(t.vertices, row[i]);
+    dst[i] = make_double2(v.x, v.y);  // slots past the polygon repeat its last vertex
   }
 }
 
@@ -511,15 +509,17 @@ __global__ void __launch_bounds__(kBlock) k_bary(const double* __restrict__ pts,
   for (int k = 0; k < nw; ++k) out[i * nw + k] = w[k];
 }
 
-template <int MAXV, int STACK, bool PACKED>
+template <int MAXV, int STACK, int LEAF>
 __global__ void __launch_bounds__(kBlock) k_locate_bary(TreeView t, const double* __restrict__ pts,
                                                         int64_t n, double tol,
                                                         int64_t* __restrict__ face_out,
                                                         double* __restrict__ out) {
+  __shared__ int lds_stack[LaneStack<STACK>::kLdsInts * kBlock];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const P2 p = ld_point(pts, i);
-  const int f = locate_one<MAXV, STACK, false, PACKED>(t, p, tol);
+  LaneStack<STACK> stack(lds_stack + threadIdx.x, kBlock);
+  const int f = locate_one<MAXV, STACK, false, LEAF>(t, p, tol, stack);
   face_out[i] = f;
   const int nw = t.nmax;
   double w[MAXV];
@@ -657,7 +657,7 @@ int check_tree(const ct_tree* tree) {
   return CT_OK;
 }
 
-template <int MAXV, bool ON_EDGE, bool PACKED>
+template <int MAXV, bool ON_EDGE, int LEAF>
 struct LocateLaunch {
   const TreeView& t;
   const double* pts;
@@ -668,16 +668,16 @@ struct LocateLaunch {
   hipStream_t st;
   template <int S>
   int run() {
-    k_locate_points<MAXV, S, ON_EDGE, PACKED><<<grid_xcd(n), kBlock, 0, st>>>(t, pts, order, n, tol, out);
+    k_locate_points<MAXV, S, ON_EDGE, LEAF><<<grid_xcd(n), kBlock, 0, st>>>(t, pts, order, n, tol, out);
     CT_LAUNCH_CHECK();
     return CT_OK;
   }
 };
 
-template <int MAXV, bool ON_EDGE, bool PACKED>
+template <int MAXV, bool ON_EDGE, int LEAF>
 int launch_locate_v(const TreeView& t, int depth, const double* pts, QueryOrder order, int64_t n,
                     double tol, int64_t* out, hipStream_t st) {
-  return with_stack(depth, LocateLaunch<MAXV, ON_EDGE, PACKED>{t, pts, order, n, tol, out, st});
+  return with_stack(depth, LocateLaunch<MAXV, ON_EDGE, LEAF>{t, pts, order, n, tol, out, st});
 }
 
 // Face trees: picks the vertex-count class, and the packed leaf path when present.
@@ -686,15 +686,15 @@ int launch_locate_faces(const ct_tree* tree, const double* pts, QueryOrder order
   const TreeView t = view(tree);
   const int nm = tree->n_max_vert, d = tree->depth;
   if (t.leaf != nullptr && nm <= 8) {
-    if (nm <= 3) return launch_locate_v<3, false, true>(t, d, pts, order, n, tol, out, st);
-    if (nm <= 4) return launch_locate_v<4, false, true>(t, d, pts, order, n, tol, out, st);
-    return launch_locate_v<8, false, true>(t, d, pts, order, n, tol, out, st);
+    if (nm <= 3) return launch_locate_v<3, false, kLeafPackedExact>(t, d, pts, order, n, tol, out, st);
+    if (nm <= 4) return launch_locate_v<4, false, kLeafPackedExact>(t, d, pts, order, n, tol, out, st);
+    return launch_locate_v<8, false, kLeafPacked>(t, d, pts, order, n, tol, out, st);
   }
-  if (nm <= 3) return launch_locate_v<3, false, false>(t, d, pts, order, n, tol, out, st);
-  if (nm <= 4) return launch_locate_v<4, false, false>(t, d, pts, order, n, tol, out, st);
-  if (nm <= 8) return launch_locate_v<8, false, false>(t, d, pts, order, n, tol, out, st);
-  if (nm <= 16) return launch_locate_v<16, false, false>(t, d, pts, order, n, tol, out, st);
-  return launch_locate_v<32, false, false>(t, d, pts, order, n, tol, out, st);
+  if (nm <= 3) return launch_locate_v<3, false, kLeafFaces>(t, d, pts, order, n, tol, out, st);
+  if (nm <= 4) return launch_locate_v<4, false, kLeafFaces>(t, d, pts, order, n, tol, out, st);
+  if (nm <= 8) return launch_locate_v<8, false, kLeafFaces>(t, d, pts, order, n, tol, out, st);
+  if (nm <= 16) return launch_locate_v<16, false, kLeafFaces>(t, d, pts, order, n, tol, out, st);
+  return launch_locate_v<32, false, kLeafFaces>(t, d, pts, order, n, tol, out, st);
 }
 
 template <bool WRITE>
@@ -753,7 +753,7 @@ int launch_edges(const ct_tree* tree, const double* segs, QueryOrder order, int6
   return with_stack(d, EdgesLaunch<WRITE, 32, true>{t, segs, order, n, counts, offsets, ii, jj, xy, st});
 }
 
-template <int MAXV, bool PACKED>
+template <int MAXV, int LEAF>
 struct LocateBaryLaunch {
   const TreeView& t;
   const double* pts;
@@ -764,7 +764,7 @@ struct LocateBaryLaunch {
   hipStream_t st;
   template <int S>
   int run() {
-    k_locate_bary<MAXV, S, PACKED><<<grid_for(n), kBlock, 0, st>>>(t, pts, n, tol, fo, w);
+    k_locate_bary<MAXV, S, LEAF><<<grid_for(n), kBlock, 0, st>>>(t, pts, n, tol, fo, w);
     CT_LAUNCH_CHECK();
     return CT_OK;
   }
@@ -871,7 +871,7 @@ int ct_locate_points_on_edge(const ct_tree* tree, const double* points, int64_t 
   CT_REQUIRE(n >= 0, "n must be >= 0");
   CT_REQUIRE(tree->n_max_vert == 2, "edge tree expected (n_max_vert == 2)");
   if (n == 0) return CT_OK;
-  return launch_locate_v<4, true, false>(view(tree), tree->depth, points, QueryOrder{nullptr, nullptr}, n, tol,
+  return launch_locate_v<4, true, kLeafFaces>(view(tree), tree->depth, points, QueryOrder{nullptr, nullptr}, n, tol,
                                          out, as_stream(stream));
 }
 
@@ -961,15 +961,15 @@ int ct_compute_barycentric_weights(const ct_tree* tree, const double* points, in
   hipStream_t st = as_stream(stream);
   const int nm = tree->n_max_vert, d = tree->depth;
   if (t.leaf != nullptr && nm <= 8) {
-    if (nm <= 3) return with_stack(d, LocateBaryLaunch<3, true>{t, points, n, tol, face_out, weights, st});
-    if (nm <= 4) return with_stack(d, LocateBaryLaunch<4, true>{t, points, n, tol, face_out, weights, st});
-    return with_stack(d, LocateBaryLaunch<8, true>{t, points, n, tol, face_out, weights, st});
+    if (nm <= 3) return with_stack(d, LocateBaryLaunch<3, kLeafPackedExact>{t, points, n, tol, face_out, weights, st});
+    if (nm <= 4) return with_stack(d, LocateBaryLaunch<4, kLeafPackedExact>{t, points, n, tol, face_out, weights, st});
+    return with_stack(d, LocateBaryLaunch<8, kLeafPacked>{t, points, n, tol, face_out, weights, st});
   }
-  if (nm <= 3) return with_stack(d, LocateBaryLaunch<3, false>{t, points, n, tol, face_out, weights, st});
-  if (nm <= 4) return with_stack(d, LocateBaryLaunch<4, false>{t, points, n, tol, face_out, weights, st});
-  if (nm <= 8) return with_stack(d, LocateBaryLaunch<8, false>{t, points, n, tol, face_out, weights, st});
-  if (nm <= 16) return with_stack(d, LocateBaryLaunch<16, false>{t, points, n, tol, face_out, weights, st});
-  return with_stack(d, LocateBaryLaunch<32, false>{t, points, n, tol, face_out, weights, st});
+  if (nm <= 3) return with_stack(d, LocateBaryLaunch<3, kLeafFaces>{t, points, n, tol, face_out, weights, st});
+  if (nm <= 4) return with_stack(d, LocateBaryLaunch<4, kLeafFaces>{t, points, n, tol, face_out, weights, st});
+  if (nm <= 8) return with_stack(d, LocateBaryLaunch<8, kLeafFaces>{t, points, n, tol, face_out, weights, st});
+  if (nm <= 16) return with_stack(d, LocateBaryLaunch<16, kLeafFaces>{t, points, n, tol, face_out, weights, st});
+  return with_stack(d, LocateBaryLaunch<32, kLeafFaces>{t, points, n, tol, face_out, weights, st});
 }
 
 int ct_collect_node_bounds(const ct_node* nodes, int64_t n_nodes, const double* bbox,

@@ -47,6 +47,35 @@ int with_stack(int depth, F&& f) {
   return f.template run<256>();
 }
 
+// Per-lane traversal stack.  Classes up to kLdsStackMax entries live in LDS, lane-interleaved
+// (slot s of lane l at base[s * stride + l]), so the pushes and pops of a wavefront touch 64
+// consecutive dwords and the stack costs no VGPRs: a VGPR array indexed by a per-lane depth
+// compiles to a select over every entry per push and pop, and its registers cut occupancy.
+// Deeper classes keep a private array (scratch memory).
+constexpr int kLdsStackMax = 32;
+
+template <int STACK, bool IN_LDS = (STACK <= kLdsStackMax)>
+struct LaneStack;
+
+template <int STACK>
+struct LaneStack<STACK, true> {
+  static constexpr int kLdsInts = STACK;  // LDS ints per lane
+  int* base;
+  int stride;
+  __device__ LaneStack(int* lds_lane, int stride_) : base(lds_lane), stride(stride_) {}
+  __device__ __forceinline__ void put(int i, int v) { base[i * stride] = v; }
+  __device__ __forceinline__ int get(int i) const { return base[i * stride]; }
+};
+
+template <int STACK>
+struct LaneStack<STACK, false> {
+  static constexpr int kLdsInts = 1;
+  int s[STACK];
+  __device__ LaneStack(int*, int) {}
+  __device__ __forceinline__ void put(int i, int v) { s[i] = v; }
+  __device__ __forceinline__ int get(int i) const { return s[i]; }
+};
+
 struct NodeLd {
   double Lmax, Rmin;
   int child, ptr, size, dim;
@@ -76,30 +105,42 @@ struct QueryOrder {
   const double* sorted;
 };
 
-// Row k of the packed leaf coordinates: the cell's vertices, unused slots NaN.  The length
-// rule is polygon_length's (geometry_utils.py:72-79): the first empty slot at index >= 3.
-template <int MAXV>
-__device__ __forceinline__ int load_leaf_poly(const double* __restrict__ leaf, int nmax, int k,
-                                              P2 (&poly)[MAXV]) {
+// Row k of the packed leaf coordinates: the cell's polygon (polygon_length's rule,
+// geometry_utils.py:72-79) in slots [0, len), slots [len, nmax) repeating vertex len - 1.
+// Registers past nmax (MAXV > nmax) repeat slot nmax - 1 as well, so `poly` always holds a
+// MAXV-vertex row in point_in_cell's padded form.
+// EXACT: the caller knows nmax == MAXV (rows of compile-time width, no selects).
+template <int MAXV, bool EXACT>
+__device__ __forceinline__ void load_leaf_poly(const double* __restrict__ leaf, int nmax, int k,
+                                               P2 (&poly)[MAXV]) {
+  if constexpr (EXACT) {
+    const double2* row = reinterpret_cast<const double2*>(leaf) + (int64_t)k * MAXV;
+#pragma unroll
+    for (int i = 0; i < MAXV; ++i) {
+      const double2 d = row[i];
+      poly[i] = {d.x, d.y};
+    }
+    return;
+  }
   const double2* row = reinterpret_cast<const double2*>(leaf) + (int64_t)k * nmax;
 #pragma unroll
   for (int i = 0; i < MAXV; ++i) {
-    if (i < nmax) {
+    if (i < 3 || i < nmax) {
       const double2 d = row[i];
       poly[i] = {d.x, d.y};
     } else {
-      poly[i] = {0.0, 0.0};
+      poly[i] = poly[i - 1];
     }
   }
-  int len = nmax < MAXV ? nmax : MAXV;
-#pragma unroll
-  for (int i = MAXV - 1; i >= 3; --i)
-    if (i < nmax && isnan(poly[i].x)) len = i;
-  return len;
 }
 
 // ----------------------------------------------------------------- points
 // query.py:62-107 locate_point (faces) / :120-164 locate_point_on_edge (edges).
+//
+// LEAF says where a leaf's cells come from: the faces and vertices (kLeafFaces), the packed
+// rows (kLeafPacked, rows of width nmax <= MAXV), or packed rows known to be exactly MAXV
+// wide (kLeafPackedExact).
+enum : int { kLeafFaces = 0, kLeafPacked = 1, kLeafPackedExact = 2 };
 //
 // Loop shape: an inner loop walks internal nodes until the lane stands on a leaf (or its
 // stack runs dry), and only then tests that leaf's cells.  Lanes of a wavefront reach
@@ -107,7 +148,7 @@ __device__ __forceinline__ int load_leaf_poly(const double* __restrict__ leaf, i
 // (masked) once for every iteration in which any lane sits on a leaf.  Here lanes that
 // reach a leaf early wait in the inner loop's exit, so the wavefront runs the cell test
 // for all of them together.  Each lane still visits its nodes in the reference's order.
-template <int MAXV, int STACK, bool ON_EDGE, bool PACKED>
+template <int MAXV, int STACK, bool ON_EDGE, int LEAF>
 __device__ __forceinline__ bool leaf_hit(const TreeView& t, P2 p, double tol, const NodeLd& nd, int* found) {
   for (int k = nd.ptr; k < nd.ptr + nd.size; ++k) {
     if constexpr (ON_EDGE) {
@@ -117,10 +158,10 @@ __device__ __forceinline__ bool leaf_hit(const TreeView& t, P2 p, double tol, co
         *found = c;
         return true;
       }
-    } else if constexpr (PACKED) {
+    } else if constexpr (LEAF != kLeafFaces) {
       P2 poly[MAXV];
-      const int len = load_leaf_poly<MAXV>(t.leaf, t.nmax, k, poly);
-      if (point_in_poly_or_edge<MAXV>(p, poly, len, tol)) {
+      load_leaf_poly<MAXV, LEAF == kLeafPackedExact>(t.leaf, t.nmax, k, poly);
+      if (point_in_cell<MAXV>(p, poly, tol)) {
         *found = t.bb_indices[k];
         return true;
       }
@@ -137,9 +178,8 @@ __device__ __forceinline__ bool leaf_hit(const TreeView& t, P2 p, double tol, co
   return false;
 }
 
-template <int MAXV, int STACK, bool ON_EDGE, bool PACKED>
-__device__ __forceinline__ int locate_one(const TreeView& t, P2 p, double tol) {
-  int stack[STACK];
+template <int MAXV, int STACK, bool ON_EDGE, int LEAF>
+__device__ __forceinline__ int locate_one(const TreeView& t, P2 p, double tol, LaneStack<STACK>& stack) {
   int sp = 0;
   int node = 0;
   while (true) {
@@ -153,7 +193,7 @@ __device__ __forceinline__ int locate_one(const TreeView& t, P2 p, double tol) {
       if (left && right) {
         // Both viable: the side with the larger margin goes first (query.py:93-101).
         const bool right_first = (nd.Lmax - v) < (v - nd.Rmin);
-        stack[sp++] = right_first ? lc : rc;
+        stack.put(sp++, right_first ? lc : rc);
         node = right_first ? rc : lc;
       } else if (left) {
         node = lc;
@@ -164,15 +204,15 @@ __device__ __forceinline__ int locate_one(const TreeView& t, P2 p, double tol) {
           exhausted = true;
           break;
         }
-        node = stack[--sp];
+        node = stack.get(--sp);
       }
       nd = ld_node(t.nodes, node);
     }
     if (exhausted) return -1;
     int found;
-    if (leaf_hit<MAXV, STACK, ON_EDGE, PACKED>(t, p, tol, nd, &found)) return found;
+    if (leaf_hit<MAXV, STACK, ON_EDGE, LEAF>(t, p, tol, nd, &found)) return found;
     if (sp == 0) return -1;
-    node = stack[--sp];
+    node = stack.get(--sp);
   }
 }
 

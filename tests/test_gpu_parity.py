@@ -140,6 +140,69 @@ def test_locate_points_every_path(trees):
         g._ctree.leaf_coords = saved
 
 
+def _degenerate_mesh(m, width, seed):
+    """Jittered m x m quad grid whose cells are written in degenerate ways: plain quads,
+    fill-padded triangle pairs, and rows with repeated vertices (at the start, middle and
+    end), in a face array of `width` columns."""
+    rng = np.random.default_rng(seed)
+    g = np.stack(np.meshgrid(np.arange(m + 1.0), np.arange(m + 1.0), indexing="xy"), -1).reshape(-1, 2)
+    v = g + rng.uniform(-0.2, 0.2, size=g.shape)
+    rows = []
+    for j in range(m):
+        for i in range(m):
+            a = j * (m + 1) + i
+            b, c, d = a + 1, a + m + 2, a + m + 1
+            kind = (i * 7 + j * 3) % 5
+            if kind == 0:
+                rows += [[a, b, c, d]]
+            elif kind == 1:
+                rows += [[a, b, c], [a, c, d]]
+            elif kind == 2:
+                rows += [[a, b, b, c], [a, c, d, d]]
+            elif kind == 3:
+                rows += [[a, a, b, c], [c, d, a, c]]
+            else:
+                rows += [[a, b, c, c, d]] if width > 4 else [[a, b, c, d]]
+    f = np.full((len(rows), width), -1, dtype=np.int64)
+    for r, row in enumerate(rows):
+        f[r, : len(row)] = row
+    return v, f
+
+
+@pytest.mark.parametrize("width", [4, 6])
+def test_locate_points_degenerate_cells(width):
+    """Packed rows padded with a repeated vertex (and rows with repeated vertices of their
+    own) answer exactly as the reference's point_in_polygon_or_on_edge loop."""
+    import torch
+
+    import numba_celltree_amd as M
+    from numba_celltree_amd import _device as D
+    from numba_celltree_amd import _native
+
+    v, f = _degenerate_mesh(40, width, seed=width)
+    g = M.CellTree2d(v, f, -1)
+    o = R.OracleCellTree2d(v, f, -1)
+    assert g._ctree.leaf_coords is not None
+    rng = np.random.default_rng(11)
+    fa = np.where(f >= 0, f, f[:, :1])
+    mid = 0.5 * (v[fa[:, 0]] + v[fa[:, 1]])
+    pts = np.concatenate([rng.uniform(-1.0, 41.0, size=(50_000, 2)), v, mid, v[fa[:, :3]].mean(1)])
+    log(f"degenerate cells width={width} faces={len(f)} points={len(pts)}")
+    d_pts = torch.from_numpy(pts).cuda()
+    saved = g._ctree.leaf_coords
+    try:
+        for tol in (None, 1e-9, 0.05):
+            want = o.locate_points(pts, tol)
+            t = g._default_tolerance() if tol is None else tol
+            for packed in (True, False):
+                g._ctree.leaf_coords = saved if packed else None
+                out = D.empty(len(pts), torch.int64)
+                _native.call("ct_locate_points", g.tree_ref, D.ptr(d_pts), len(pts), t, D.ptr(out), D.stream())
+                assert np.array_equal(out.cpu().numpy(), want), (tol, packed)
+    finally:
+        g._ctree.leaf_coords = saved
+
+
 @pytest.mark.parametrize("kind", [0, 1, 2])
 def test_query_order_is_permutation(kind):
     """ct_query_order returns a permutation, also for clustered, outside and NaN queries."""
