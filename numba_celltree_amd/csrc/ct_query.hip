@@ -103,23 +103,32 @@ __device__ __forceinline__ unsigned order_key(int kind, const double* __restrict
   return ((unsigned)cls << 16) | (m & 0xffffu);
 }
 
-// Two-level MSD bucket sort on the 20-bit key, without global atomics:
-//   k_sort_hist     per tile of kSortTile queries: LDS histogram of the top 8 key bits,
-//                   written bin-major to T[bin][tile]
-//   scan of T       -> the start of every (bin, tile) run
-//   k_sort_scatter  per tile: LDS cursors, (key, index) written to the coarse runs
-//   k_sort_fine     one workgroup per coarse bin: LDS histogram / scan / scatter of the
-//                   low 12 key bits -> the final order
+// Two-level bucket sort on the 20-bit key, without global atomics:
+//   k_sort_hist     per tile of kSortTile queries: the keys (written out) and an LDS
+//                   histogram of their top 8 bits, stored bin-major in T[bin][tile]
+//   scan of T       -> Ts, the start of every (bin, tile) run of the coarse order
+//   k_sort_scatter  per tile: the tile's (key, index) pairs grouped by coarse bin in LDS,
+//                   then copied out as one contiguous run per bin (coalesced stores)
+//   k_sort_chunks   per chunk of at most kChunk entries of one coarse bin: LDS histogram,
+//                   scan and ranks of the low 12 key bits -> that stretch of the order
+// Every stage reads and writes each entry once.  A coarse bin (1/256 of the bbox) larger
+// than kChunk is ordered chunk by chunk, so the order is exact within chunks of kChunk
+// queries and interleaves them within the bin.  Order only schedules lanes: the traversal
+// needs a wavefront's queries to be close, and 64 consecutive entries of a 16384-query
+// chunk of one coarse cell still span about 1/256 of that cell.
 constexpr int kSortItems = 32;
 constexpr int kSortBatch = 8;  // loads in flight per lane
 constexpr int kSortTile = kBlock * kSortItems;
 constexpr int kCoarse = 256;  // == kBlock
 constexpr int kFineBits = 12;
 constexpr int kFine = 1 << kFineBits;
-constexpr int kFineBlock = 1024;
+constexpr int kChunkBlock = 1024;
+constexpr int kChunkItems = 16;  // entries per lane, held in registers
+constexpr int kChunk = kChunkBlock * kChunkItems;
 
 __global__ void __launch_bounds__(kBlock) k_sort_hist(int kind, const double* __restrict__ q, int64_t n,
                                                       OrderGeom g, int64_t ntiles,
+                                                      unsigned* __restrict__ keys,
                                                       int64_t* __restrict__ T) {
   __shared__ int hist[kCoarse];
   hist[threadIdx.x] = 0;
@@ -133,8 +142,12 @@ __global__ void __launch_bounds__(kBlock) k_sort_hist(int kind, const double* __
       k[j] = (i < n) ? order_key(kind, q, i, g) : ~0u;
     }
 #pragma unroll
-    for (int j = 0; j < kSortBatch; ++j)
-      if (k[j] != ~0u) atomicAdd(&hist[k[j] >> kFineBits], 1);
+    for (int j = 0; j < kSortBatch; ++j) {
+      if (k[j] != ~0u) {
+        atomicAdd(&hist[k[j] >> kFineBits], 1);
+        keys[base + (int64_t)(j0 + j) * kBlock + threadIdx.x] = k[j];
+      }
+    }
   }
   __syncthreads();
   T[(int64_t)threadIdx.x * ntiles + blockIdx.x] = hist[threadIdx.x];
@@ -152,90 +165,119 @@ __device__ __forceinline__ void copy_row(int kind, const double* __restrict__ fr
 }
 
 __global__ void __launch_bounds__(kBlock) k_sort_scatter(int kind, const double* __restrict__ q, int64_t n,
-                                                         OrderGeom g, int64_t ntiles,
+                                                         int64_t ntiles, const unsigned* __restrict__ keys_in,
+                                                         const int64_t* __restrict__ T,
                                                          const int64_t* __restrict__ Ts,
                                                          unsigned* __restrict__ keys,
                                                          int32_t* __restrict__ idx, double* __restrict__ cq) {
+  __shared__ unsigned sk[kSortTile];
+  __shared__ int si[kSortTile];
   __shared__ int cur[kCoarse];
-  cur[threadIdx.x] = (int)Ts[(int64_t)threadIdx.x * ntiles + blockIdx.x];
+  __shared__ int64_t shift[kCoarse];  // global start - tile-local start of every bin's run
+  __shared__ int lds[kBlock / 64];
+  const int64_t tile = blockIdx.x;
+  const int b = threadIdx.x;
+  int total;
+  const int local = block_exclusive_scan<int, kBlock>((int)T[(int64_t)b * ntiles + tile], lds, &total);
+  cur[b] = local;
+  shift[b] = Ts[(int64_t)b * ntiles + tile] - local;
   __syncthreads();
-  const int64_t base = (int64_t)blockIdx.x * kSortTile;
+  const int64_t base = tile * kSortTile;
   for (int j0 = 0; j0 < kSortItems; j0 += kSortBatch) {
     unsigned k[kSortBatch];
 #pragma unroll
     for (int j = 0; j < kSortBatch; ++j) {
       const int64_t i = base + (int64_t)(j0 + j) * kBlock + threadIdx.x;
-      k[j] = (i < n) ? order_key(kind, q, i, g) : ~0u;
+      k[j] = (i < n) ? keys_in[i] : ~0u;
     }
 #pragma unroll
     for (int j = 0; j < kSortBatch; ++j) {
       if (k[j] != ~0u) {
         const int pos = atomicAdd(&cur[k[j] >> kFineBits], 1);
-        const int64_t i = base + (int64_t)(j0 + j) * kBlock + threadIdx.x;
-        keys[pos] = k[j];
-        idx[pos] = (int32_t)i;
-        if (cq) copy_row(kind, q, i, cq, pos);
+        sk[pos] = k[j];
+        si[pos] = (j0 + j) * kBlock + threadIdx.x;
       }
     }
+  }
+  __syncthreads();
+  // LDS slot pos of bin c goes to shift[c] + pos: consecutive slots, consecutive addresses.
+  for (int pos = threadIdx.x; pos < total; pos += kBlock) {
+    const unsigned key = sk[pos];
+    const int64_t dst = shift[key >> kFineBits] + pos;
+    const int64_t i = base + si[pos];
+    keys[dst] = key;
+    idx[dst] = (int32_t)i;
+    if (cq) copy_row(kind, q, i, cq, dst);
   }
 }
 
-__global__ void __launch_bounds__(kFineBlock) k_sort_fine(int kind, int64_t ntiles, const int64_t* __restrict__ Ts,
-                                                          int64_t n, const unsigned* __restrict__ keys,
-                                                          const int32_t* __restrict__ idx,
-                                                          const double* __restrict__ cq,
-                                                          int32_t* __restrict__ order,
-                                                          double* __restrict__ sorted) {
+__global__ void __launch_bounds__(kChunkBlock) k_sort_chunks(int kind, int64_t ntiles, const int64_t* __restrict__ Ts,
+                                                             int64_t n, const unsigned* __restrict__ keys,
+                                                             const int32_t* __restrict__ idx,
+                                                             const double* __restrict__ cq,
+                                                             int32_t* __restrict__ order,
+                                                             double* __restrict__ sorted) {
   __shared__ int hist[kFine];
-  __shared__ int lds[kFineBlock / 64];
-  const int cb = blockIdx.x;
-  const int lo = (int)Ts[(int64_t)cb * ntiles];
-  const int hi = (cb + 1 < kCoarse) ? (int)Ts[(int64_t)(cb + 1) * ntiles] : (int)n;
-  for (int b = threadIdx.x; b < kFine; b += kFineBlock) hist[b] = 0;
-  __syncthreads();
-  constexpr int B = 4;
-  for (int p0 = lo + threadIdx.x; p0 < hi; p0 += B * kFineBlock) {
-    unsigned k[B];
-#pragma unroll
-    for (int j = 0; j < B; ++j) {
-      const int p = p0 + j * kFineBlock;
-      k[j] = (p < hi) ? keys[p] : ~0u;
-    }
-#pragma unroll
-    for (int j = 0; j < B; ++j)
-      if (k[j] != ~0u) atomicAdd(&hist[k[j] & (kFine - 1)], 1);
+  __shared__ int staged[kChunk];
+  __shared__ int lds[kChunkBlock / 64];
+  __shared__ int64_t span[2];
+  // Chunk c of the grid: coarse bins in order, ceil(count / kChunk) chunks each.
+  int64_t blo = 0, bhi = 0;
+  int nch = 0;
+  if (threadIdx.x < kCoarse) {
+    const int c = threadIdx.x;
+    blo = Ts[(int64_t)c * ntiles];
+    bhi = (c + 1 < kCoarse) ? Ts[(int64_t)(c + 1) * ntiles] : n;
+    nch = (int)((bhi - blo + kChunk - 1) / kChunk);
   }
+  if (threadIdx.x == 0) span[0] = span[1] = 0;
+  int all;
+  const int first = block_exclusive_scan<int, kChunkBlock>(nch, lds, &all);
+  if ((int)blockIdx.x >= first && (int)blockIdx.x < first + nch) {
+    const int64_t lo = blo + (int64_t)((int)blockIdx.x - first) * kChunk;
+    span[0] = lo;
+    span[1] = lo + kChunk < bhi ? lo + kChunk : bhi;
+  }
+  for (int f = threadIdx.x; f < kFine; f += kChunkBlock) hist[f] = 0;
   __syncthreads();
-  constexpr int per = kFine / kFineBlock;
+  const int64_t lo = span[0], hi = span[1];
+  if (lo >= hi) return;  // past the last chunk (the grid is an upper bound); block-uniform
+  unsigned k[kChunkItems];
+#pragma unroll
+  for (int j = 0; j < kChunkItems; ++j) {
+    const int64_t p = lo + (int64_t)j * kChunkBlock + threadIdx.x;
+    k[j] = p < hi ? keys[p] : ~0u;
+  }
+#pragma unroll
+  for (int j = 0; j < kChunkItems; ++j)
+    if (k[j] != ~0u) atomicAdd(&hist[k[j] & (kFine - 1)], 1);
+  __syncthreads();
+  constexpr int per = kFine / kChunkBlock;
   int s = 0;
 #pragma unroll
-  for (int k = 0; k < per; ++k) s += hist[threadIdx.x * per + k];
+  for (int f = 0; f < per; ++f) s += hist[threadIdx.x * per + f];
   int total;
-  int run = lo + block_exclusive_scan<int, kFineBlock>(s, lds, &total);
+  int run = block_exclusive_scan<int, kChunkBlock>(s, lds, &total);
 #pragma unroll
-  for (int k = 0; k < per; ++k) {
-    const int c = hist[threadIdx.x * per + k];
-    hist[threadIdx.x * per + k] = run;
+  for (int f = 0; f < per; ++f) {
+    const int c = hist[threadIdx.x * per + f];
+    hist[threadIdx.x * per + f] = run;
     run += c;
   }
   __syncthreads();
-  for (int p0 = lo + threadIdx.x; p0 < hi; p0 += B * kFineBlock) {
-    unsigned k[B];
-    int v[B];
 #pragma unroll
-    for (int j = 0; j < B; ++j) {
-      const int p = p0 + j * kFineBlock;
-      k[j] = (p < hi) ? keys[p] : ~0u;
-      v[j] = (p < hi) ? idx[p] : 0;
+  for (int j = 0; j < kChunkItems; ++j) {
+    if (k[j] != ~0u) {
+      const int pos = atomicAdd(&hist[k[j] & (kFine - 1)], 1);
+      staged[pos] = j * kChunkBlock + threadIdx.x;
     }
-#pragma unroll
-    for (int j = 0; j < B; ++j) {
-      if (k[j] != ~0u) {
-        const int pos = atomicAdd(&hist[k[j] & (kFine - 1)], 1);
-        order[pos] = v[j];
-        if (sorted) copy_row(kind, cq, p0 + j * kFineBlock, sorted, pos);
-      }
-    }
+  }
+  __syncthreads();
+  const int m = (int)(hi - lo);
+  for (int pos = threadIdx.x; pos < m; pos += kChunkBlock) {
+    const int64_t src = lo + staged[pos];
+    order[lo + pos] = idx[src];
+    if (sorted) copy_row(kind, cq, src, sorted, lo + pos);
   }
 }
 
@@ -244,7 +286,8 @@ struct OrderWs {
   int64_t* Ts;
   void* scan_ws;
   size_t scan_bytes;
-  unsigned* keys;
+  unsigned* kraw;  // keys in query order
+  unsigned* keys;  // keys in coarse-bin order
   int32_t* idx;
   double* cq;      // query rows in coarse-bin order (up to 4 doubles per query)
   int32_t* order;  // only used by ct_locate_points_ordered
@@ -269,6 +312,7 @@ OrderWs carve_order(char* base, int64_t n, bool with_order, size_t* total) {
   w.Ts = reinterpret_cast<int64_t*>(take(sizeof(int64_t) * (nt + 1)));
   w.scan_bytes = ct_scan_workspace_bytes(nt);
   w.scan_ws = take(w.scan_bytes);
+  w.kraw = reinterpret_cast<unsigned*>(take(sizeof(unsigned) * nn));
   w.keys = reinterpret_cast<unsigned*>(take(sizeof(unsigned) * nn));
   w.idx = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * nn));
   w.cq = reinterpret_cast<double*>(take(sizeof(double) * 4 * nn));
@@ -288,14 +332,16 @@ int query_order(int kind, const double* q, int64_t n, const double* bbox, const 
   g.sy = 1.0 / ey;
   g.inv_area = 1.0 / (ex * ey);
   const unsigned ntiles = (unsigned)w.ntiles;
-  k_sort_hist<<<ntiles, kBlock, 0, st>>>(kind, q, n, g, w.ntiles, w.T);
+  k_sort_hist<<<ntiles, kBlock, 0, st>>>(kind, q, n, g, w.ntiles, w.kraw, w.T);
   CT_LAUNCH_CHECK();
   int rc = exclusive_scan(w.T, (int64_t)kCoarse * w.ntiles, w.Ts, w.scan_ws, w.scan_bytes, st);
   if (rc) return rc;
   double* cq = sorted ? w.cq : nullptr;
-  k_sort_scatter<<<ntiles, kBlock, 0, st>>>(kind, q, n, g, w.ntiles, w.Ts, w.keys, w.idx, cq);
+  k_sort_scatter<<<ntiles, kBlock, 0, st>>>(kind, q, n, w.ntiles, w.kraw, w.T, w.Ts, w.keys, w.idx, cq);
   CT_LAUNCH_CHECK();
-  k_sort_fine<<<kCoarse, kFineBlock, 0, st>>>(kind, w.ntiles, w.Ts, n, w.keys, w.idx, cq, order, sorted);
+  // sum over bins of ceil(count / kChunk) <= ceil(n / kChunk) + kCoarse
+  const unsigned nchunks = (unsigned)((n + kChunk - 1) / kChunk + kCoarse);
+  k_sort_chunks<<<nchunks, kChunkBlock, 0, st>>>(kind, w.ntiles, w.Ts, n, w.keys, w.idx, cq, order, sorted);
   CT_LAUNCH_CHECK();
   return CT_OK;
 }

@@ -235,6 +235,54 @@ def test_query_order_is_permutation(kind):
     assert np.array_equal(srt.cpu().numpy().view(np.int64), rows.view(np.int64))
 
 
+def _point_keys(q, bbox):
+    """Host restatement of the 20-bit point key: Morton code of a 1024 x 1024 grid over bbox."""
+    def cell(x, lo, hi):
+        g = (x - lo) * (1.0 / (hi - lo)) * 1024.0
+        g = np.where(g > 0.0, np.where(g < 1023.0, g, 1023.0), 0.0)  # NaN -> 0
+        return g.astype(np.uint32)
+
+    def spread(v):
+        out = np.zeros_like(v)
+        for b in range(10):
+            out |= ((v >> b) & 1) << (2 * b)
+        return out
+
+    return spread(cell(q[:, 0], bbox[0], bbox[1])) | (spread(cell(q[:, 1], bbox[2], bbox[3])) << 1)
+
+
+def test_query_order_groups_and_chunks():
+    """The point order is grouped by the top 8 key bits and sorted by the full key within
+    every chunk of 16384 entries of a group; a clustered set makes groups of many chunks."""
+    import torch
+
+    from numba_celltree_amd import _device as D
+
+    rng = np.random.default_rng(5)
+    bbox = [0.0, 100.0, 0.0, 100.0]
+    q = np.concatenate([
+        rng.uniform(-5, 105, size=(150_000, 2)),
+        rng.normal(30.0, 0.5, size=(70_000, 2)),  # ~4 chunks in one or two coarse groups
+        rng.uniform(0, 100, size=(1, 2)),
+    ])
+    q[17] = np.nan
+    n = len(q)
+    keys = _point_keys(q, bbox)
+    order = D.query_order(0, torch.from_numpy(q).cuda(), bbox).cpu().numpy()
+    assert np.array_equal(np.sort(order), np.arange(n))
+    k = keys[order]
+    coarse = k >> 12
+    assert np.all(np.diff(coarse.astype(np.int64)) >= 0)
+    n_chunks = 0
+    for c in np.unique(coarse):
+        lo, hi = np.searchsorted(coarse, c, "left"), np.searchsorted(coarse, c, "right")
+        for s in range(lo, hi, 16384):
+            seg = k[s:min(s + 16384, hi)]
+            assert np.all(np.diff(seg.astype(np.int64)) >= 0), (c, s)
+            n_chunks += 1
+    assert n_chunks > len(np.unique(coarse))  # some group holds several chunks
+
+
 def test_locate_boxes_ordered(trees):
     name, v, f, g, o = trees
     L = _extent(v)
