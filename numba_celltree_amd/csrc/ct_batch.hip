@@ -29,6 +29,7 @@ constexpr int kWave = 64;
 // threads (4 waves per SIMD), caps a lane at 128 VGPRs and spills the segment traversal to
 // scratch; with 256 (one wave per SIMD) it may use what the traversal needs.
 constexpr int kBatchMaxBlock = 256;
+static_assert(kBatchMaxBlock == kStackStride, "batch workgroups are at most kStackStride lanes");
 
 struct BatchArgs {
   const ct_tree* trees;
@@ -57,8 +58,7 @@ __device__ __forceinline__ int next_tile(const BatchArgs& a, int& queue, int& mi
 
 template <int MAXV, int STACK>
 __global__ void __launch_bounds__(kBatchMaxBlock) k_batch_points(BatchArgs a, const double* __restrict__ pts, int64_t* __restrict__ out) {
-  __shared__ int lds_stack[LaneStack<STACK>::kLdsInts * kBatchMaxBlock];
-  LaneStack<STACK> stack(lds_stack + threadIdx.x, kBatchMaxBlock);
+  CT_LANE_STACK(STACK, stack);
   int queue = blockIdx.x % a.n_queues, misses = 0;
   const int lane = threadIdx.x & (kWave - 1);
   for (int tid = next_tile(a, queue, misses); tid >= 0; tid = next_tile(a, queue, misses)) {

@@ -31,12 +31,11 @@ template <int MAXV, int STACK, bool ON_EDGE, int LEAF>
 __global__ void __launch_bounds__(kBlock) k_locate_points(TreeView t, const double* __restrict__ pts,
                                                           QueryOrder ord, int64_t n, double tol,
                                                           int64_t* __restrict__ out) {
-  __shared__ int lds_stack[LaneStack<STACK>::kLdsInts * kBlock];
   const int64_t i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int64_t q = query_index(ord.order, i);
   const P2 p = ord.sorted ? ld_point(ord.sorted, i) : ld_point(pts, q);
-  LaneStack<STACK> stack(lds_stack + threadIdx.x, kBlock);
+  CT_LANE_STACK(STACK, stack);
   out[q] = locate_one<MAXV, STACK, ON_EDGE, LEAF>(t, p, tol, stack);
 }
 
@@ -560,11 +559,10 @@ __global__ void __launch_bounds__(kBlock) k_locate_bary(TreeView t, const double
                                                         int64_t n, double tol,
                                                         int64_t* __restrict__ face_out,
                                                         double* __restrict__ out) {
-  __shared__ int lds_stack[LaneStack<STACK>::kLdsInts * kBlock];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const P2 p = ld_point(pts, i);
-  LaneStack<STACK> stack(lds_stack + threadIdx.x, kBlock);
+  CT_LANE_STACK(STACK, stack);
   const int f = locate_one<MAXV, STACK, false, LEAF>(t, p, tol, stack);
   face_out[i] = f;
   const int nw = t.nmax;

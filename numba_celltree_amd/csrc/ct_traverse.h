@@ -47,34 +47,44 @@ int with_stack(int depth, F&& f) {
   return f.template run<256>();
 }
 
-// Per-lane traversal stack.  Classes up to kLdsStackMax entries live in LDS, lane-interleaved
-// (slot s of lane l at base[s * stride + l]), so the pushes and pops of a wavefront touch 64
-// consecutive dwords and the stack costs no VGPRs: a VGPR array indexed by a per-lane depth
-// compiles to a select over every entry per push and pop, and its registers cut occupancy.
-// Deeper classes keep a private array (scratch memory).
-constexpr int kLdsStackMax = 32;
-
-template <int STACK, bool IN_LDS = (STACK <= kLdsStackMax)>
-struct LaneStack;
-
-template <int STACK>
-struct LaneStack<STACK, true> {
-  static constexpr int kLdsInts = STACK;  // LDS ints per lane
-  int* base;
-  int stride;
-  __device__ LaneStack(int* lds_lane, int stride_) : base(lds_lane), stride(stride_) {}
-  __device__ __forceinline__ void put(int i, int v) { base[i * stride] = v; }
-  __device__ __forceinline__ int get(int i) const { return base[i * stride]; }
-};
+// Per-lane traversal stack of STACK entries.  The first kLdsStackSlots entries live in LDS,
+// lane-interleaved (slot s of lane l at base[s * stride + l]: a wavefront's pushes and pops
+// touch 64 consecutive dwords); deeper entries go to a private array in scratch memory,
+// which point queries rarely reach.  A VGPR array indexed by a per-lane depth would compile
+// to a select over every entry per push and pop and cost occupancy; a full LDS stack
+// (STACK ints per lane) would cap the workgroups per CU.  With 4 LDS slots the traversal
+// kernels keep 8 wavefronts per SIMD.
+//
+// The kernel owns both arrays and hands this view two pointers, which stay in registers, so
+// the compiler keeps LDS accesses as ds_* and spill accesses as scratch_* instructions (a
+// struct holding the array itself would live in scratch and turn both into flat accesses).
+constexpr int kLdsStackSlots = 4;
+constexpr int kStackStride = 256;  // lanes per workgroup of every traversal kernel
+static_assert(kBlock == kStackStride, "single-tree traversal workgroups are kStackStride lanes");
 
 template <int STACK>
-struct LaneStack<STACK, false> {
-  static constexpr int kLdsInts = 1;
-  int s[STACK];
-  __device__ LaneStack(int*, int) {}
-  __device__ __forceinline__ void put(int i, int v) { s[i] = v; }
-  __device__ __forceinline__ int get(int i) const { return s[i]; }
+struct LaneStack {
+  static_assert(STACK > kLdsStackSlots, "stack classes start at 16");
+  static constexpr int kSpill = STACK - kLdsStackSlots;
+  int* base;             // this lane's slot 0 in the workgroup's LDS array
+  volatile int* spill;   // kSpill private ints (volatile: scratch, not VGPRs)
+  __device__ __forceinline__ void put(int i, int v) {
+    if (i < kLdsStackSlots) base[i * kStackStride] = v;
+    else spill[i - kLdsStackSlots] = v;
+  }
+  __device__ __forceinline__ int get(int i) {
+    int v;
+    if (i < kLdsStackSlots) v = base[i * kStackStride];
+    else v = spill[i - kLdsStackSlots];
+    return v;
+  }
 };
+
+// Declares the LDS and private arrays of a traversal stack and its view `name`.
+#define CT_LANE_STACK(STACK, name)                                \
+  __shared__ int name##_lds[kLdsStackSlots * kStackStride];       \
+  volatile int name##_spill[LaneStack<STACK>::kSpill];            \
+  LaneStack<STACK> name { name##_lds + threadIdx.x, name##_spill }
 
 struct NodeLd {
   double Lmax, Rmin;
