@@ -41,6 +41,25 @@ struct BatchArgs {
   int32_t* counters;
 };
 
+// Element i of a read-only device array, for a wave-uniform i: through the constant address
+// space the load becomes a scalar load into SGPRs.  A tile's descriptor, tree, tolerance
+// and query offset are the same for the whole wavefront; held in VGPRs they cost ~12
+// registers per lane and a wavefront per SIMD of occupancy.
+// Structs are read as 32-bit words and reassembled.
+template <typename T>
+__device__ __forceinline__ T ld_uniform(const T* p, int64_t i) {
+  static_assert(sizeof(T) % 4 == 0 && std::is_trivially_copyable<T>::value, "plain records of whole words");
+  constexpr int kWords = (int)(sizeof(T) / 4);
+  const __attribute__((address_space(4))) uint32_t* src =
+      (const __attribute__((address_space(4))) uint32_t*)(p + i);
+  uint32_t w[kWords];
+#pragma unroll
+  for (int k = 0; k < kWords; ++k) w[k] = src[k];
+  T r;
+  __builtin_memcpy(&r, w, sizeof(T));
+  return r;
+}
+
 // Next tile for this wavefront, or -1 when every queue is drained.  Wave-uniform.
 __device__ __forceinline__ int next_tile(const BatchArgs& a, int& queue, int& misses) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -62,15 +81,20 @@ __global__ void __launch_bounds__(kBatchMaxBlock) k_batch_points(BatchArgs a, co
   int queue = blockIdx.x % a.n_queues, misses = 0;
   const int lane = threadIdx.x & (kWave - 1);
   for (int tid = next_tile(a, queue, misses); tid >= 0; tid = next_tile(a, queue, misses)) {
-    const ct_batch_tile tile = a.tiles[tid];
-    const TreeView t = view(a.trees[tile.tree]);
-    const double tol = a.tols[tile.tree];
-    const int64_t base = a.qoff[tile.tree] + tile.begin;
+    const ct_batch_tile tile = ld_uniform(a.tiles, __builtin_amdgcn_readfirstlane(tid));
+    const int m = __builtin_amdgcn_readfirstlane(tile.tree);
+    const TreeView t = view(ld_uniform(a.trees, m));
+    const double tol = ld_uniform(a.tols, m);
+    const int64_t base = ld_uniform(a.qoff, m) + tile.begin;
     for (int j = lane; j < tile.count; j += kWave) {
       const int64_t q = base + j;
       const P2 p = ld_point(pts, q);
-      out[q] = t.leaf ? locate_one<MAXV, STACK, false, kLeafPacked>(t, p, tol, stack)
-                      : locate_one<MAXV, STACK, false, kLeafFaces>(t, p, tol, stack);
+      // wave-uniform choice of the leaf-row layout of this tree
+      int r;
+      if (t.leaf == nullptr) r = locate_one<MAXV, STACK, false, kLeafFaces>(t, p, tol, stack);
+      else if (t.nmax == MAXV) r = locate_one<MAXV, STACK, false, kLeafPackedExact>(t, p, tol, stack);
+      else r = locate_one<MAXV, STACK, false, kLeafPacked>(t, p, tol, stack);
+      out[q] = r;
     }
   }
 }
@@ -82,20 +106,22 @@ __global__ void __launch_bounds__(kBatchMaxBlock) k_batch_edges(BatchArgs a, con
                               const int64_t* __restrict__ offsets, int64_t capacity,
                               int64_t* __restrict__ ii, int64_t* __restrict__ jj,
                               double* __restrict__ xy) {
+  CT_LANE_STACK(STACK, stack);
   int queue = blockIdx.x % a.n_queues, misses = 0;
   const int lane = threadIdx.x & (kWave - 1);
   for (int tid = next_tile(a, queue, misses); tid >= 0; tid = next_tile(a, queue, misses)) {
-    const ct_batch_tile tile = a.tiles[tid];
-    const TreeView t = view(a.trees[tile.tree]);
-    const int64_t base = a.qoff[tile.tree] + tile.begin;
+    const ct_batch_tile tile = ld_uniform(a.tiles, __builtin_amdgcn_readfirstlane(tid));
+    const int m = __builtin_amdgcn_readfirstlane(tile.tree);
+    const TreeView t = view(ld_uniform(a.trees, m));
+    const int64_t base = ld_uniform(a.qoff, m) + tile.begin;
     for (int j = lane; j < tile.count; j += kWave) {
       const int64_t q = base + j;
       const double4 s4 = reinterpret_cast<const double4*>(segs)[q];
       const P2 pa = {s4.x, s4.y}, pb = {s4.z, s4.w};
       if (WRITE) {
-        if (offsets[q + 1] <= capacity) edge_query<true, MAXV, STACK, true>(t, pa, pb, q, offsets[q], ii, jj, xy);
+        if (offsets[q + 1] <= capacity) edge_query<true, MAXV, STACK, true>(t, pa, pb, q, offsets[q], ii, jj, xy, stack);
       } else {
-        counts[q] = edge_query<false, MAXV, STACK, true>(t, pa, pb, q, 0, nullptr, nullptr, nullptr);
+        counts[q] = edge_query<false, MAXV, STACK, true>(t, pa, pb, q, 0, nullptr, nullptr, nullptr, stack);
       }
     }
   }

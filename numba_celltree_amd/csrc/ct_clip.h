@@ -193,98 +193,5 @@ __device__ __forceinline__ double clip_area(const P2 (&subj)[MA], int ls, const 
   }
 }
 
-// ------------------------------------------------------------------ edge labels
-// The same clip, also recording for every output vertex k the input line its outgoing edge
-// (k -> k + 1) lies on: subject edge i (subj[i] -> subj[i + 1], label i) or the clip edge of
-// stage j (clip[j - 1] -> clip[j], label kClipLabel + j).  Rules per input edge a -> b:
-// an entering intersection starts an edge along a -> b; an exiting intersection starts an
-// edge along the clip line; a copied vertex keeps its own outgoing label.  A skipped
-// zero-length edge hands its label on to the next edge (they lie on the same point).
-constexpr int kClipLabel = 64;
-
-template <int CAP>
-__device__ __forceinline__ int sh_stage_labels(const P2 (&in)[CAP], const int (&lab)[CAP], int len, P2 r, P2 s,
-                                               int clip_label, P2 (&out)[CAP], int (&olab)[CAP]) {
-  const double ux = s.x - r.x, uy = s.y - r.y;
-  const double nx = -uy, ny = ux;
-  int n_out = 0;
-  P2 a = get(in, len - 1);
-  int alab = get(lab, len - 1);
-  bool a_in = sh_in(a, r, ux, uy);
-#pragma unroll
-  for (int j = 0; j < CAP; ++j) {
-    if (j < len) {
-      const P2 b = in[j];
-      const int blab = lab[j];
-      const double vx = b.x - a.x, vy = b.y - a.y;
-      if (vx == 0 && vy == 0) {
-        alab = blab;
-      } else {
-        bool b_in = sh_in(b, r, ux, uy);
-        const double nv = nx * vx + ny * vy;
-        bool cross_pt = false, copy_b = false;
-        int xlab = alab;
-        if (b_in) {
-          cross_pt = !a_in && nv != 0;
-          copy_b = true;
-        } else if (a_in) {
-          if (nv != 0) {
-            cross_pt = true;
-            xlab = clip_label;
-          } else {
-            b_in = true;
-            copy_b = true;
-          }
-        }
-        if (cross_pt && n_out < CAP) {
-          const double nw = nx * (r.x - a.x) + ny * (r.y - a.y);
-          const double t = nw / nv;
-          put(out, n_out, P2{a.x + t * vx, a.y + t * vy});
-          put(olab, n_out, xlab);
-          ++n_out;
-        }
-        if (copy_b && n_out < CAP) {
-          put(out, n_out, b);
-          put(olab, n_out, blab);
-          ++n_out;
-        }
-        a = b;
-        alab = blab;
-        a_in = b_in;
-      }
-    }
-  }
-  return n_out;
-}
-
-// The clipped polygon with edge labels; returns its length (< 3: empty).
-template <int MA, int MB, int CAP>
-__device__ __forceinline__ int sh_run_labels(const P2 (&subj)[MA], int ls, const P2 (&clip)[MB], int lc,
-                                             P2 (&poly)[CAP], int (&lab)[CAP]) {
-#pragma unroll
-  for (int i = 0; i < CAP; ++i) {
-    poly[i] = (i < MA) ? subj[i < MA ? i : 0] : P2{0.0, 0.0};
-    lab[i] = i;
-  }
-  int len = ls;
-  P2 r = get(clip, lc - 1);
-  for (int ci = 0; ci < lc; ++ci) {
-    const P2 s = get(clip, ci);
-    if (!(s.x - r.x == 0 && s.y - r.y == 0)) {
-      P2 tmp[CAP];
-      int tlab[CAP];
-      len = sh_stage_labels<CAP>(poly, lab, len, r, s, kClipLabel + ci, tmp, tlab);
-#pragma unroll
-      for (int i = 0; i < CAP; ++i) {
-        poly[i] = tmp[i];
-        lab[i] = tlab[i];
-      }
-      if (len < 3) return len;
-    }
-    r = s;
-  }
-  return len;
-}
-
 }  // namespace
 }  // namespace ct

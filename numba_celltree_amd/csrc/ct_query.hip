@@ -373,10 +373,12 @@ __global__ void __launch_bounds__(kBlock) k_boxes(TreeView t, const double* __re
   if (i >= n) return;
   const int64_t q = query_index(ord.order, i);
   const Box box = ord.sorted ? ld_box(ord.sorted, i) : ld_box(boxes, q);
+  // Box queries keep deep stacks (a box often overlaps both children), which would spill
+  // past LaneStack's LDS slots on most queries: a VGPR array is faster here.
+  int stack[STACK];
   int64_t cnt = 0;
   int64_t o = WRITE ? offsets[q] : 0;
   if (boxes_intersect(box, t.bbox)) {
-    int stack[STACK];
     int sp = 0;
     int node = 0;
     // Internal nodes in an inner loop; the leaf's cells once the whole wavefront is at a
@@ -437,10 +439,11 @@ __global__ void __launch_bounds__(kBlock) k_edges(TreeView t, const double* __re
   const double4 s4 = ord.sorted ? reinterpret_cast<const double4*>(ord.sorted)[i]
                                 : reinterpret_cast<const double4*>(segs)[q];
   const P2 a = {s4.x, s4.y}, b = {s4.z, s4.w};
+  CT_LANE_STACK(STACK, stack);
   if (WRITE) {
-    edge_query<true, MAXV, STACK, FACES>(t, a, b, q, offsets[q], ii, jj, xy);
+    edge_query<true, MAXV, STACK, FACES>(t, a, b, q, offsets[q], ii, jj, xy, stack);
   } else {
-    counts[q] = edge_query<false, MAXV, STACK, FACES>(t, a, b, q, 0, nullptr, nullptr, nullptr);
+    counts[q] = edge_query<false, MAXV, STACK, FACES>(t, a, b, q, 0, nullptr, nullptr, nullptr, stack);
   }
 }
 

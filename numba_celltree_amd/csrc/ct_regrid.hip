@@ -5,14 +5,15 @@
 //
 // A_k is the Sutherland-Hodgman overlap area of pair k (subject = target polygon t_k,
 // clipper = source polygon s_k; sutherland_hodgman.py:84-148) and area_t the fan area of
-// the target polygon (geometry_utils.py:82-95).  The pair topology (which pairs, and the
-// path each clip takes) is held fixed; the adjoint differentiates the arithmetic of that
-// path with respect to F, the source vertices and the target vertices.
+// the target polygon (geometry_utils.py:82-95).  The pair topology (which pairs overlap) is
+// held fixed; the adjoint is the analytic derivative of the overlap areas and target areas
+// with respect to F, the source vertices and the target vertices.
 //
 // Pair lists are grouped by target (ascending t, as intersect_faces returns them): the
 // forward is a deterministic segmented sum per target.  The transpose uses a by-source
-// permutation of the pairs (also deterministic).  Vertex gradients are scattered with FP64
-// atomics (summation order may vary between runs; each contribution is exact).
+// permutation of the pairs (also deterministic).  Vertex gradients are reduced without
+// atomics, in a fixed order (per-pair corner rows, per-face corner sums, per-vertex CSR
+// gathers), so they are identical from run to run.
 //
 // Every polygon lives in registers (ct_clip.h): loops run to the compile-time capacity with
 // a length guard and runtime-indexed accesses go through select chains, so no kernel here
@@ -116,57 +117,118 @@ __global__ void __launch_bounds__(kBlock) k_pair_areas(const double* __restrict_
 // Per pair: gA = sum_f gy[t,f] F[s,f] / area_t, times the derivative of the overlap area.
 //
 // With both polygons convex and counter-clockwise, the clip computes their intersection P,
-// and under a fixed clip path its area is a smooth function of the input vertices whose
-// derivative lives on the boundary of P: every edge of P lies on one input edge v0 -> v1
-// (d = v1 - v0), covering the parameter range [alpha, beta] of it, and moving v0 (v1) moves
-// the point at parameter u by (1 - u) (u) times the displacement.  The area changes by the
-// outward normal velocity integrated over the covered part, so with m = (d.y, -d.x)
-// (outward normal times |d|):
+// whose area is a smooth function of the input vertices away from degenerate contacts; its
+// derivative lives on the boundary of P.  That boundary is made of the pieces of the input
+// edges that lie in the other polygon: every input edge v0 -> v1 (d = v1 - v0) contributes
+// the parameter range [alpha, beta] of it that lies in the other polygon, and moving v0 (v1)
+// moves the point at parameter u by (1 - u) (u) times the displacement.  The area changes by
+// the outward normal velocity integrated over that range, so with m = (d.y, -d.x) (outward
+// normal times |d|):
 //   dA/dv0 += m * ((beta - alpha) - (beta^2 - alpha^2) / 2),   dA/dv1 += m * (beta^2 - alpha^2) / 2.
-// The clip records the input edge under every edge of P (sh_run_labels), so one clip pass
-// gives the whole gradient.  It equals the derivative of the stage-by-stage arithmetic
-// (checked against finite differences and an autograd restatement of the clip).
-template <int MA, int MB, int CAP>
+// The ranges come from clipping each edge of one polygon against the half-planes of the
+// other (Cyrus-Beck), with every index static: no clipped polygon, no runtime-indexed
+// registers.  Where an edge of the target (the clip's subject) lies on an edge line of the
+// source (the clipper), the Sutherland-Hodgman pass counts that boundary piece as the
+// source's edge (its inside test is strict, sutherland_hodgman.py:57-60): target edges are
+// clipped against open half-planes, source edges against closed ones, and a source edge on
+// a target edge line counts only when both run the same way (overlapping interiors; when
+// they run opposite ways the cells only touch and the clip has no polygon).
+// Checked against finite differences, torch.autograd.gradcheck and an autograd restatement
+// of the clip.
+
+// Shrinks [t0, t1] to the part of v0 + t d inside the line through r with direction u
+// (inside: u x (p - r) > 0, sh_in's test).  CLOSED: on the line counts as inside when the
+// segment runs the same way as u.
+template <bool CLOSED>
+__device__ __forceinline__ void clip_range(P2 v0, double dx, double dy, P2 r, double ux, double uy,
+                                           double& t0, double& t1) {
+  const double c0 = ux * (v0.y - r.y) - uy * (v0.x - r.x);
+  const double c1 = ux * dy - uy * dx;
+  if (c1 == 0.0) {
+    bool in = c0 > 0.0;
+    if (CLOSED && c0 == 0.0) in = (ux * dx + uy * dy) > 0.0;
+    if (!in) t1 = -1.0;
+  } else {
+    const double tc = -c0 / c1;
+    if (c1 > 0.0) {
+      t0 = tc > t0 ? tc : t0;
+    } else {
+      t1 = tc < t1 ? tc : t1;
+    }
+  }
+}
+
+// The two vertex contributions of the range [alpha, beta] of edge d (times gA).
+__device__ __forceinline__ void edge_range_gradient(double alpha, double beta, double dx, double dy, double gA,
+                                                    P2& g0, P2& g1) {
+  const double w1 = 0.5 * (beta * beta - alpha * alpha);
+  const double w0 = (beta - alpha) - w1;
+  g0 = {gA * w0 * dy, -gA * w0 * dx};
+  g1 = {gA * w1 * dy, -gA * w1 * dx};
+}
+
+template <int MA, int MB>
 __device__ __forceinline__ void overlap_area_gradient(const P2 (&a)[MA], int la, const P2 (&b)[MB], int lb,
-                                                      const P2 (&poly)[CAP], const int (&lab)[CAP], int len,
                                                       double gA, P2 (&ga)[MA], P2 (&gb)[MB]) {
+  const P2 b_last = get(b, lb - 1);
+  // target edges a[i] -> a[i + 1 mod la] against the source's open half-planes
 #pragma unroll
-  for (int k = 0; k < CAP; ++k) {
-    if (k < len) {
-      const P2 p = poly[k];
-      const P2 q = (k + 1 < len) ? poly[(k + 1 < CAP) ? k + 1 : 0] : poly[0];
-      const int L = lab[k];
-      const bool on_subject = L < kClipLabel;
-      P2 v0, v1;
-      int i0, i1;
-      if (on_subject) {
-        i0 = L;
-        i1 = (L + 1 == la) ? 0 : L + 1;
-        v0 = get(a, i0);
-        v1 = get(a, i1);
-      } else {  // stage j clips against clip[j - 1] -> clip[j]
-        const int j = L - kClipLabel;
-        i0 = (j == 0) ? lb - 1 : j - 1;
-        i1 = j;
-        v0 = get(b, i0);
-        v1 = get(b, i1);
-      }
+  for (int i = 0; i < MA; ++i) {
+    if (i < la) {
+      const P2 v0 = a[i];
+      const P2 v1 = (i + 1 < la) ? a[(i + 1 < MA) ? i + 1 : 0] : a[0];
       const double dx = v1.x - v0.x, dy = v1.y - v0.y;
-      const double dd = dx * dx + dy * dy;
-      if (dd > 0.0) {
-        const double alpha = ((p.x - v0.x) * dx + (p.y - v0.y) * dy) / dd;
-        const double beta = ((q.x - v0.x) * dx + (q.y - v0.y) * dy) / dd;
-        const double w1 = 0.5 * (beta * beta - alpha * alpha);
-        const double w0 = (beta - alpha) - w1;
-        const P2 g0 = {gA * w0 * dy, -gA * w0 * dx};
-        const P2 g1 = {gA * w1 * dy, -gA * w1 * dx};
-        if (on_subject) {
-          add_at(ga, i0, g0);
-          add_at(ga, i1, g1);
-        } else {
-          add_at(gb, i0, g0);
-          add_at(gb, i1, g1);
+      double t0 = 0.0, t1 = 1.0;
+#pragma unroll
+      for (int j = 0; j < MB; ++j) {
+        if (j < lb) {
+          const P2 r = (j == 0) ? b_last : b[(j > 0) ? j - 1 : 0];
+          clip_range<false>(v0, dx, dy, r, b[j].x - r.x, b[j].y - r.y, t0, t1);
         }
+      }
+      if (t1 > t0) {
+        P2 g0, g1;
+        edge_range_gradient(t0, t1, dx, dy, gA, g0, g1);
+        ga[i].x += g0.x;
+        ga[i].y += g0.y;
+        const bool wrap = !(i + 1 < la);
+        if (i + 1 < MA) {
+          ga[(i + 1 < MA) ? i + 1 : 0].x += wrap ? 0.0 : g1.x;
+          ga[(i + 1 < MA) ? i + 1 : 0].y += wrap ? 0.0 : g1.y;
+        }
+        ga[0].x += wrap ? g1.x : 0.0;
+        ga[0].y += wrap ? g1.y : 0.0;
+      }
+    }
+  }
+  // source edges b[j - 1 mod lb] -> b[j] against the target's closed half-planes
+#pragma unroll
+  for (int j = 0; j < MB; ++j) {
+    if (j < lb) {
+      const P2 v0 = (j == 0) ? b_last : b[(j > 0) ? j - 1 : 0];
+      const P2 v1 = b[j];
+      const double dx = v1.x - v0.x, dy = v1.y - v0.y;
+      double t0 = 0.0, t1 = 1.0;
+#pragma unroll
+      for (int i = 0; i < MA; ++i) {
+        if (i < la) {
+          const P2 r = a[i];
+          const P2 s = (i + 1 < la) ? a[(i + 1 < MA) ? i + 1 : 0] : a[0];
+          clip_range<true>(v0, dx, dy, r, s.x - r.x, s.y - r.y, t0, t1);
+        }
+      }
+      if (t1 > t0) {
+        P2 g0, g1;
+        edge_range_gradient(t0, t1, dx, dy, gA, g0, g1);
+        // g0 goes to vertex j - 1 mod lb, g1 to vertex j
+        if (j > 0) {
+          gb[(j > 0) ? j - 1 : 0].x += g0.x;
+          gb[(j > 0) ? j - 1 : 0].y += g0.y;
+        } else {
+          add_at(gb, lb - 1, g0);
+        }
+        gb[j].x += g1.x;
+        gb[j].y += g1.y;
       }
     }
   }
@@ -191,7 +253,6 @@ __global__ void __launch_bounds__(kBlock) k_pair_backward(
   double gA = 0.0;
   for (int f = 0; f < nf; ++f) gA += gy[t * nf + f] * F[s * nf + f];
   gA /= tarea[t];
-  constexpr int CAP = MA + MB;
   P2 ga[MA], gb[MB];
 #pragma unroll
   for (int i = 0; i < MA; ++i) ga[i] = {0.0, 0.0};
@@ -202,10 +263,7 @@ __global__ void __launch_bounds__(kBlock) k_pair_backward(
     int ai[MA], bi[MB];
     const int la = load_face_idx<MA>(tf, ntm, t, tv, a, ai);
     const int lb = load_face_idx<MB>(sf, nsm, s, sv, b, bi);
-    P2 poly[CAP];
-    int lab[CAP];
-    const int len = sh_run_labels<MA, MB, CAP>(a, la, b, lb, poly, lab);
-    if (len >= 3) overlap_area_gradient<MA, MB, CAP>(a, la, b, lb, poly, lab, len, gA, ga, gb);
+    overlap_area_gradient<MA, MB>(a, la, b, lb, gA, ga, gb);
   }
   double2* ra = reinterpret_cast<double2*>(GA) + k * ntm;
   double2* rb = reinterpret_cast<double2*>(GB) + k * nsm;

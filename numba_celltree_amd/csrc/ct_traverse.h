@@ -296,7 +296,7 @@ __device__ __forceinline__ bool seg_meets_box(P2 a, double dx, double dy, double
 template <bool WRITE, int MAXV, int STACK, bool FACES>
 __device__ __forceinline__ int64_t edge_query(const TreeView& t, P2 a, P2 b, int64_t q, int64_t o,
                                               int64_t* __restrict__ ii, int64_t* __restrict__ jj,
-                                              double* __restrict__ xy) {
+                                              double* __restrict__ xy, LaneStack<STACK>& stack) {
   int64_t cnt = 0;
   P2 c0, d0;
   if (!cohen_sutherland(a, b, t.bbox, &c0, &d0)) return 0;
@@ -307,7 +307,6 @@ __device__ __forceinline__ int64_t edge_query(const TreeView& t, P2 a, P2 b, int
   const double4* __restrict__ nbox = prune ? reinterpret_cast<const double4*>(t.nbox) + 1 : nullptr;
   // Same loop shape as locate_one: internal nodes in an inner loop, the leaf's cell tests
   // once every lane of the wavefront stands on a leaf or has finished.
-  int stack[STACK];
   int sp = 0;
   int node = 0;
   while (true) {
@@ -340,7 +339,7 @@ __device__ __forceinline__ int64_t edge_query(const TreeView& t, P2 a, P2 b, int
         if (right) right = seg_meets_box(a, vx, vy, mseg, nbox[nd.child + 1]);
       }
       if (left && right) {
-        stack[sp++] = nd.child;
+        stack.put(sp++, nd.child);
         node = nd.child + 1;
       } else if (left || right) {
         node = nd.child + (right ? 1 : 0);
@@ -349,7 +348,7 @@ __device__ __forceinline__ int64_t edge_query(const TreeView& t, P2 a, P2 b, int
           exhausted = true;
           break;
         }
-        node = stack[--sp];
+        node = stack.get(--sp);
       }
       nd = ld_node(t.nodes, node);
     }
@@ -368,7 +367,7 @@ __device__ __forceinline__ int64_t edge_query(const TreeView& t, P2 a, P2 b, int
       }
     }
     if (sp == 0) break;
-    node = stack[--sp];
+    node = stack.get(--sp);
   }
   return cnt;
 }
