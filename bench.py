@@ -133,11 +133,13 @@ def extras(args):
             return rg(F, sv_t, tv_t)
 
     def fwd_bwd():
+        # G is the output gradient (the gradient of sum(y * G)), handed to the backward
+        # directly so the leg times the regrid forward and backward only, no loss kernels.
         Fg = F.detach().requires_grad_(True)
         svg = sv_t.detach().requires_grad_(True)
         tvg = tv_t.detach().requires_grad_(True)
         y = rg(Fg, svg, tvg)
-        (y * G).sum().backward()
+        y.backward(G)
         return Fg.grad, svg.grad, tvg.grad
 
     for name, fn in (("regrid_fwd_16f_ms", fwd), ("regrid_fwd_bwd_16f_ms", fwd_bwd)):

@@ -295,8 +295,9 @@ int ct_regrid_apply(const int64_t* tgt_offsets, const int64_t* ss, const double*
                     const double* tgt_area, const double* F, int32_t n_fields, int64_t n_tgt,
                     double* y, void* stream);
 
-/* gF (n_src, n_fields) = transpose, over perm (pairs sorted by source) and src_offsets. */
-int ct_regrid_apply_transpose(const int64_t* src_offsets, const int64_t* perm, const int64_t* tt,
+/* gF (n_src, n_fields) = transpose, over perm (pairs sorted by source) and src_offsets;
+ * src_tgt[m] = tt[perm[m]], the pairs' targets in that order. */
+int ct_regrid_apply_transpose(const int64_t* src_offsets, const int64_t* perm, const int64_t* src_tgt,
                               const double* area, const double* tgt_area, const double* gy,
                               int32_t n_fields, int64_t n_src, double* gF, void* stream);
 

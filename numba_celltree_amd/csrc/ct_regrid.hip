@@ -136,24 +136,36 @@ __global__ void __launch_bounds__(kBlock) k_pair_areas(const double* __restrict_
 // Checked against finite differences, torch.autograd.gradcheck and an autograd restatement
 // of the clip.
 
-// Shrinks [t0, t1] to the part of v0 + t d inside the line through r with direction u
+// Parameter range [n0 / d0, n1 / d1] of a segment (d0, d1 > 0), kept as fractions so that
+// clipping compares by cross-multiplication and divides once per end at the close.
+struct EdgeRange {
+  double n0 = 0.0, d0 = 1.0, n1 = 1.0, d1 = 1.0;
+  bool empty = false;
+};
+
+// Shrinks the range to the part of v0 + t d inside the line through r with direction u
 // (inside: u x (p - r) > 0, sh_in's test).  CLOSED: on the line counts as inside when the
-// segment runs the same way as u.
+// segment runs the same way as u.  The bound of a crossing line is -c0 / c1.
 template <bool CLOSED>
 __device__ __forceinline__ void clip_range(P2 v0, double dx, double dy, P2 r, double ux, double uy,
-                                           double& t0, double& t1) {
+                                           EdgeRange& e) {
   const double c0 = ux * (v0.y - r.y) - uy * (v0.x - r.x);
   const double c1 = ux * dy - uy * dx;
   if (c1 == 0.0) {
     bool in = c0 > 0.0;
     if (CLOSED && c0 == 0.0) in = (ux * dx + uy * dy) > 0.0;
-    if (!in) t1 = -1.0;
-  } else {
-    const double tc = -c0 / c1;
-    if (c1 > 0.0) {
-      t0 = tc > t0 ? tc : t0;
-    } else {
-      t1 = tc < t1 ? tc : t1;
+    e.empty = e.empty || !in;
+  } else if (c1 > 0.0) {  // t >= -c0 / c1
+    const double n = -c0;
+    if (n * e.d0 > e.n0 * c1) {
+      e.n0 = n;
+      e.d0 = c1;
+    }
+  } else {  // t <= c0 / -c1
+    const double d = -c1;
+    if (c0 * e.d1 < e.n1 * d) {
+      e.n1 = c0;
+      e.d1 = d;
     }
   }
 }
@@ -178,15 +190,16 @@ __device__ __forceinline__ void overlap_area_gradient(const P2 (&a)[MA], int la,
       const P2 v0 = a[i];
       const P2 v1 = (i + 1 < la) ? a[(i + 1 < MA) ? i + 1 : 0] : a[0];
       const double dx = v1.x - v0.x, dy = v1.y - v0.y;
-      double t0 = 0.0, t1 = 1.0;
+      EdgeRange e;
 #pragma unroll
       for (int j = 0; j < MB; ++j) {
         if (j < lb) {
           const P2 r = (j == 0) ? b_last : b[(j > 0) ? j - 1 : 0];
-          clip_range<false>(v0, dx, dy, r, b[j].x - r.x, b[j].y - r.y, t0, t1);
+          clip_range<false>(v0, dx, dy, r, b[j].x - r.x, b[j].y - r.y, e);
         }
       }
-      if (t1 > t0) {
+      const double t0 = e.n0 / e.d0, t1 = e.n1 / e.d1;
+      if (!e.empty && t1 > t0) {
         P2 g0, g1;
         edge_range_gradient(t0, t1, dx, dy, gA, g0, g1);
         ga[i].x += g0.x;
@@ -208,16 +221,17 @@ __device__ __forceinline__ void overlap_area_gradient(const P2 (&a)[MA], int la,
       const P2 v0 = (j == 0) ? b_last : b[(j > 0) ? j - 1 : 0];
       const P2 v1 = b[j];
       const double dx = v1.x - v0.x, dy = v1.y - v0.y;
-      double t0 = 0.0, t1 = 1.0;
+      EdgeRange e;
 #pragma unroll
       for (int i = 0; i < MA; ++i) {
         if (i < la) {
           const P2 r = a[i];
           const P2 s = (i + 1 < la) ? a[(i + 1 < MA) ? i + 1 : 0] : a[0];
-          clip_range<true>(v0, dx, dy, r, s.x - r.x, s.y - r.y, t0, t1);
+          clip_range<true>(v0, dx, dy, r, s.x - r.x, s.y - r.y, e);
         }
       }
-      if (t1 > t0) {
+      const double t0 = e.n0 / e.d0, t1 = e.n1 / e.d1;
+      if (!e.empty && t1 > t0) {
         P2 g0, g1;
         edge_range_gradient(t0, t1, dx, dy, gA, g0, g1);
         // g0 goes to vertex j - 1 mod lb, g1 to vertex j
@@ -251,7 +265,18 @@ __global__ void __launch_bounds__(kBlock) k_pair_backward(
   if (k >= n) return;
   const int64_t t = tt[k], s = ss[k];
   double gA = 0.0;
-  for (int f = 0; f < nf; ++f) gA += gy[t * nf + f] * F[s * nf + f];
+  const double* gyt = gy + t * nf;
+  const double* Fs = F + s * nf;
+  int f = 0;
+  for (; f + 4 <= nf; f += 4) {  // four fields in flight, summed in field order
+    const double y0 = gyt[f], y1 = gyt[f + 1], y2 = gyt[f + 2], y3 = gyt[f + 3];
+    const double f0 = Fs[f], f1 = Fs[f + 1], f2 = Fs[f + 2], f3 = Fs[f + 3];
+    gA += y0 * f0;
+    gA += y1 * f1;
+    gA += y2 * f2;
+    gA += y3 * f3;
+  }
+  for (; f < nf; ++f) gA += gyt[f] * Fs[f];
   gA /= tarea[t];
   P2 ga[MA], gb[MB];
 #pragma unroll
@@ -292,13 +317,39 @@ __global__ void __launch_bounds__(kBlock) k_target_corners(const double* __restr
   const int64_t t = c / nmax;
   const int slot = (int)(c - t * nmax);
   double gx = 0.0, gyv = 0.0;
-  for (int64_t k = toff[t]; k < toff[t + 1]; ++k) {
-    const double2 d = reinterpret_cast<const double2*>(GA)[k * nmax + slot];
+  const double2* ga = reinterpret_cast<const double2*>(GA) + slot;
+  // four rows in flight, summed in pair order
+  int64_t k = toff[t];
+  const int64_t end = toff[t + 1];
+  for (; k + 4 <= end; k += 4) {
+    const double2 d0 = ga[k * nmax], d1 = ga[(k + 1) * nmax], d2 = ga[(k + 2) * nmax], d3 = ga[(k + 3) * nmax];
+    gx += d0.x;
+    gyv += d0.y;
+    gx += d1.x;
+    gyv += d1.y;
+    gx += d2.x;
+    gyv += d2.y;
+    gx += d3.x;
+    gyv += d3.y;
+  }
+  for (; k < end; ++k) {
+    const double2 d = ga[k * nmax];
     gx += d.x;
     gyv += d.y;
   }
   double s = 0.0;
-  for (int k = 0; k < nf; ++k) s += gy[t * nf + k] * y[t * nf + k];
+  const double* gyt = gy + t * nf;
+  const double* yt = y + t * nf;
+  int j = 0;
+  for (; j + 4 <= nf; j += 4) {  // four fields in flight, summed in field order
+    const double g0 = gyt[j], g1 = gyt[j + 1], g2 = gyt[j + 2], g3 = gyt[j + 3];
+    const double y0 = yt[j], y1 = yt[j + 1], y2 = yt[j + 2], y3 = yt[j + 3];
+    s += g0 * y0;
+    s += g1 * y1;
+    s += g2 * y2;
+    s += g3 * y3;
+  }
+  for (; j < nf; ++j) s += gyt[j] * yt[j];
   const double garea = -s / tarea[t];
   if (garea != 0.0) {
     P2 p[MAXV];
@@ -324,10 +375,26 @@ __global__ void __launch_bounds__(kBlock) k_source_corners(int nmax, int64_t n, 
   const int64_t s = c / nmax;
   const int slot = (int)(c - s * nmax);
   double gx = 0.0, gyv = 0.0;
-  for (int64_t m = soff[s]; m < soff[s + 1]; ++m) {
-    const double2 d = reinterpret_cast<const double2*>(GB)[perm[m] * nmax + slot];
-    gx += d.x;
-    gyv += d.y;
+  const double2* gb = reinterpret_cast<const double2*>(GB) + slot;
+  const int64_t end = soff[s + 1];
+  for (int64_t m = soff[s]; m < end; m += 4) {  // chunks of four, as in k_apply_t
+    const int64_t m1 = (m + 1 < end) ? m + 1 : m, m2 = (m + 2 < end) ? m + 2 : m, m3 = (m + 3 < end) ? m + 3 : m;
+    const int64_t k0 = perm[m], k1 = perm[m1], k2 = perm[m2], k3 = perm[m3];
+    const double2 d0 = gb[k0 * nmax], d1 = gb[k1 * nmax], d2 = gb[k2 * nmax], d3 = gb[k3 * nmax];
+    gx += d0.x;
+    gyv += d0.y;
+    if (m + 1 < end) {
+      gx += d1.x;
+      gyv += d1.y;
+    }
+    if (m + 2 < end) {
+      gx += d2.x;
+      gyv += d2.y;
+    }
+    if (m + 3 < end) {
+      gx += d3.x;
+      gyv += d3.y;
+    }
   }
   reinterpret_cast<double2*>(SC)[c] = make_double2(gx, gyv);
 }
@@ -339,10 +406,28 @@ __global__ void __launch_bounds__(kBlock) k_vertex_gather(int64_t n_vertices, co
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n_vertices) return;
   double gx = 0.0, gy = 0.0;
-  for (int64_t c = voff[v]; c < voff[v + 1]; ++c) {
-    const double2 d = reinterpret_cast<const double2*>(C)[corners[c]];
-    gx += d.x;
-    gy += d.y;
+  const double2* rows = reinterpret_cast<const double2*>(C);
+  const int64_t end = voff[v + 1];
+  // chunks of four corners with every load in flight (indices past the end repeat the
+  // first of the chunk and are not added); summed in corner order
+  for (int64_t c = voff[v]; c < end; c += 4) {
+    const int64_t c1 = (c + 1 < end) ? c + 1 : c, c2 = (c + 2 < end) ? c + 2 : c, c3 = (c + 3 < end) ? c + 3 : c;
+    const int64_t i0 = corners[c], i1 = corners[c1], i2 = corners[c2], i3 = corners[c3];
+    const double2 d0 = rows[i0], d1 = rows[i1], d2 = rows[i2], d3 = rows[i3];
+    gx += d0.x;
+    gy += d0.y;
+    if (c + 1 < end) {
+      gx += d1.x;
+      gy += d1.y;
+    }
+    if (c + 2 < end) {
+      gx += d2.x;
+      gy += d2.y;
+    }
+    if (c + 3 < end) {
+      gx += d3.x;
+      gy += d3.y;
+    }
   }
   reinterpret_cast<double2*>(g_v)[v] = make_double2(gx, gy);
 }
@@ -369,13 +454,26 @@ __global__ void __launch_bounds__(kBlock) k_apply(const int64_t* __restrict__ to
   const int64_t t = i / nf;
   const int f = (int)(i - t * nf);
   double acc = 0.0;
-  for (int64_t k = toff[t]; k < toff[t + 1]; ++k) acc += area[k] * F[ss[k] * nf + f];
+  int64_t k = toff[t];
+  const int64_t end = toff[t + 1];
+  for (; k + 4 <= end; k += 4) {  // four pairs in flight, summed in pair order
+    const int64_t s0 = ss[k], s1 = ss[k + 1], s2 = ss[k + 2], s3 = ss[k + 3];
+    const double a0 = area[k], a1 = area[k + 1], a2 = area[k + 2], a3 = area[k + 3];
+    const double f0 = F[s0 * nf + f], f1 = F[s1 * nf + f], f2 = F[s2 * nf + f], f3 = F[s3 * nf + f];
+    acc += a0 * f0;
+    acc += a1 * f1;
+    acc += a2 * f2;
+    acc += a3 * f3;
+  }
+  for (; k < end; ++k) acc += area[k] * F[ss[k] * nf + f];
   y[i] = acc / tarea[t];
 }
 
 // gF[s, f] = sum over the source's pairs (perm order) of area_k / area_t gy[t_k, f].
+// src_tgt[m] = tt[perm[m]] (the pairs' targets in by-source order) makes the pair index and
+// its target two independent loads instead of a chain perm -> tt -> gy.
 __global__ void __launch_bounds__(kBlock) k_apply_t(const int64_t* __restrict__ soff, const int64_t* __restrict__ perm,
-                                                    const int64_t* __restrict__ tt, const double* __restrict__ area,
+                                                    const int64_t* __restrict__ src_tgt, const double* __restrict__ area,
                                                     const double* __restrict__ tarea, const double* __restrict__ gy,
                                                     int nf, int64_t n_s, double* __restrict__ gF) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -383,10 +481,20 @@ __global__ void __launch_bounds__(kBlock) k_apply_t(const int64_t* __restrict__ 
   const int64_t s = i / nf;
   const int f = (int)(i - s * nf);
   double acc = 0.0;
-  for (int64_t m = soff[s]; m < soff[s + 1]; ++m) {
-    const int64_t k = perm[m];
-    const int64_t t = tt[k];
-    acc += area[k] / tarea[t] * gy[t * nf + f];
+  const int64_t end = soff[s + 1];
+  // chunks of four pairs with every load in flight (indices past the end repeat the chunk's
+  // first pair and are not added); summed in perm order
+  for (int64_t m = soff[s]; m < end; m += 4) {
+    const int64_t m1 = (m + 1 < end) ? m + 1 : m, m2 = (m + 2 < end) ? m + 2 : m, m3 = (m + 3 < end) ? m + 3 : m;
+    const int64_t k0 = perm[m], k1 = perm[m1], k2 = perm[m2], k3 = perm[m3];
+    const int64_t t0 = src_tgt[m], t1 = src_tgt[m1], t2 = src_tgt[m2], t3 = src_tgt[m3];
+    const double w0 = area[k0] / tarea[t0], w1 = area[k1] / tarea[t1], w2 = area[k2] / tarea[t2],
+                 w3 = area[k3] / tarea[t3];
+    const double g0 = gy[t0 * nf + f], g1 = gy[t1 * nf + f], g2 = gy[t2 * nf + f], g3 = gy[t3 * nf + f];
+    acc += w0 * g0;
+    if (m + 1 < end) acc += w1 * g1;
+    if (m + 2 < end) acc += w2 * g2;
+    if (m + 3 < end) acc += w3 * g3;
   }
   gF[i] = acc;
 }
@@ -504,13 +612,13 @@ int ct_regrid_apply(const int64_t* tgt_offsets, const int64_t* ss, const double*
   return CT_OK;
 }
 
-int ct_regrid_apply_transpose(const int64_t* src_offsets, const int64_t* perm, const int64_t* tt,
+int ct_regrid_apply_transpose(const int64_t* src_offsets, const int64_t* perm, const int64_t* src_tgt,
                               const double* area, const double* tgt_area, const double* gy, int32_t n_fields,
                               int64_t n_src, double* gF, void* stream) {
   CT_REQUIRE(n_src >= 0 && n_fields >= 1, "bad sizes");
   if (n_src == 0) return CT_OK;
-  k_apply_t<<<grid_for(n_src * n_fields), kBlock, 0, as_stream(stream)>>>(src_offsets, perm, tt, area, tgt_area,
-                                                                          gy, n_fields, n_src, gF);
+  k_apply_t<<<grid_for(n_src * n_fields), kBlock, 0, as_stream(stream)>>>(src_offsets, perm, src_tgt, area,
+                                                                          tgt_area, gy, n_fields, n_src, gF);
   CT_LAUNCH_CHECK();
   return CT_OK;
 }

@@ -62,6 +62,7 @@ class RegridTopology:
         self.n_pairs = int(tt.numel())
         self.tgt_offsets = _offsets(tt, self.n_tgt)
         self.perm_by_src = torch.argsort(ss, stable=True)
+        self.src_tgt = tt[self.perm_by_src].contiguous()
         self.src_offsets = _offsets(ss, self.n_src)
         self.n_tgt_vertices = int(d_tv.shape[0])
         self.n_src_vertices = int(tree.d_vertices.shape[0])
@@ -103,7 +104,8 @@ class RegridTopology:
         nf = int(gy.shape[1])
         gF = D.empty((self.n_src, nf), torch.float64)
         _native.call("ct_regrid_apply_transpose", D.ptr(self.src_offsets), D.ptr(self.perm_by_src),
-                     D.ptr(self.tt), D.ptr(area), D.ptr(tarea), D.ptr(gy), nf, self.n_src, D.ptr(gF), D.stream())
+                     D.ptr(self.src_tgt), D.ptr(area), D.ptr(tarea), D.ptr(gy), nf, self.n_src, D.ptr(gF),
+                     D.stream())
         return gF
 
     def vertex_backward(self, src_v, tgt_v, F, y, gy, tarea):
