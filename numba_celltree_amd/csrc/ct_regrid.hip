@@ -376,25 +376,24 @@ __global__ void __launch_bounds__(kBlock) k_source_corners(int nmax, int64_t n, 
   const int slot = (int)(c - s * nmax);
   double gx = 0.0, gyv = 0.0;
   const double2* gb = reinterpret_cast<const double2*>(GB) + slot;
+  int64_t m = soff[s];
   const int64_t end = soff[s + 1];
-  for (int64_t m = soff[s]; m < end; m += 4) {  // chunks of four, as in k_apply_t
-    const int64_t m1 = (m + 1 < end) ? m + 1 : m, m2 = (m + 2 < end) ? m + 2 : m, m3 = (m + 3 < end) ? m + 3 : m;
-    const int64_t k0 = perm[m], k1 = perm[m1], k2 = perm[m2], k3 = perm[m3];
+  for (; m + 4 <= end; m += 4) {  // four rows in flight, summed in perm order
+    const int64_t k0 = perm[m], k1 = perm[m + 1], k2 = perm[m + 2], k3 = perm[m + 3];
     const double2 d0 = gb[k0 * nmax], d1 = gb[k1 * nmax], d2 = gb[k2 * nmax], d3 = gb[k3 * nmax];
     gx += d0.x;
     gyv += d0.y;
-    if (m + 1 < end) {
-      gx += d1.x;
-      gyv += d1.y;
-    }
-    if (m + 2 < end) {
-      gx += d2.x;
-      gyv += d2.y;
-    }
-    if (m + 3 < end) {
-      gx += d3.x;
-      gyv += d3.y;
-    }
+    gx += d1.x;
+    gyv += d1.y;
+    gx += d2.x;
+    gyv += d2.y;
+    gx += d3.x;
+    gyv += d3.y;
+  }
+  for (; m < end; ++m) {
+    const double2 d = gb[perm[m] * nmax];
+    gx += d.x;
+    gyv += d.y;
   }
   reinterpret_cast<double2*>(SC)[c] = make_double2(gx, gyv);
 }
@@ -407,27 +406,24 @@ __global__ void __launch_bounds__(kBlock) k_vertex_gather(int64_t n_vertices, co
   if (v >= n_vertices) return;
   double gx = 0.0, gy = 0.0;
   const double2* rows = reinterpret_cast<const double2*>(C);
+  int64_t c = voff[v];
   const int64_t end = voff[v + 1];
-  // chunks of four corners with every load in flight (indices past the end repeat the
-  // first of the chunk and are not added); summed in corner order
-  for (int64_t c = voff[v]; c < end; c += 4) {
-    const int64_t c1 = (c + 1 < end) ? c + 1 : c, c2 = (c + 2 < end) ? c + 2 : c, c3 = (c + 3 < end) ? c + 3 : c;
-    const int64_t i0 = corners[c], i1 = corners[c1], i2 = corners[c2], i3 = corners[c3];
+  for (; c + 4 <= end; c += 4) {  // four rows in flight, summed in corner order
+    const int64_t i0 = corners[c], i1 = corners[c + 1], i2 = corners[c + 2], i3 = corners[c + 3];
     const double2 d0 = rows[i0], d1 = rows[i1], d2 = rows[i2], d3 = rows[i3];
     gx += d0.x;
     gy += d0.y;
-    if (c + 1 < end) {
-      gx += d1.x;
-      gy += d1.y;
-    }
-    if (c + 2 < end) {
-      gx += d2.x;
-      gy += d2.y;
-    }
-    if (c + 3 < end) {
-      gx += d3.x;
-      gy += d3.y;
-    }
+    gx += d1.x;
+    gy += d1.y;
+    gx += d2.x;
+    gy += d2.y;
+    gx += d3.x;
+    gy += d3.y;
+  }
+  for (; c < end; ++c) {
+    const double2 d = rows[corners[c]];
+    gx += d.x;
+    gy += d.y;
   }
   reinterpret_cast<double2*>(g_v)[v] = make_double2(gx, gy);
 }
@@ -481,20 +477,23 @@ __global__ void __launch_bounds__(kBlock) k_apply_t(const int64_t* __restrict__ 
   const int64_t s = i / nf;
   const int f = (int)(i - s * nf);
   double acc = 0.0;
+  int64_t m = soff[s];
   const int64_t end = soff[s + 1];
-  // chunks of four pairs with every load in flight (indices past the end repeat the chunk's
-  // first pair and are not added); summed in perm order
-  for (int64_t m = soff[s]; m < end; m += 4) {
-    const int64_t m1 = (m + 1 < end) ? m + 1 : m, m2 = (m + 2 < end) ? m + 2 : m, m3 = (m + 3 < end) ? m + 3 : m;
-    const int64_t k0 = perm[m], k1 = perm[m1], k2 = perm[m2], k3 = perm[m3];
-    const int64_t t0 = src_tgt[m], t1 = src_tgt[m1], t2 = src_tgt[m2], t3 = src_tgt[m3];
+  for (; m + 4 <= end; m += 4) {  // four pairs in flight, summed in perm order
+    const int64_t k0 = perm[m], k1 = perm[m + 1], k2 = perm[m + 2], k3 = perm[m + 3];
+    const int64_t t0 = src_tgt[m], t1 = src_tgt[m + 1], t2 = src_tgt[m + 2], t3 = src_tgt[m + 3];
     const double w0 = area[k0] / tarea[t0], w1 = area[k1] / tarea[t1], w2 = area[k2] / tarea[t2],
                  w3 = area[k3] / tarea[t3];
     const double g0 = gy[t0 * nf + f], g1 = gy[t1 * nf + f], g2 = gy[t2 * nf + f], g3 = gy[t3 * nf + f];
     acc += w0 * g0;
-    if (m + 1 < end) acc += w1 * g1;
-    if (m + 2 < end) acc += w2 * g2;
-    if (m + 3 < end) acc += w3 * g3;
+    acc += w1 * g1;
+    acc += w2 * g2;
+    acc += w3 * g3;
+  }
+  for (; m < end; ++m) {
+    const int64_t k = perm[m];
+    const int64_t t = src_tgt[m];
+    acc += area[k] / tarea[t] * gy[t * nf + f];
   }
   gF[i] = acc;
 }
