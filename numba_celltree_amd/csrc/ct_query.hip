@@ -241,11 +241,15 @@ __global__ void __launch_bounds__(kChunkBlock) k_sort_chunks(int kind, int64_t n
   __syncthreads();
   const int64_t lo = span[0], hi = span[1];
   if (lo >= hi) return;  // past the last chunk (the grid is an upper bound); block-uniform
+  // v: the query index of each entry, staged by rank so the order is written with one
+  // coalesced copy; with `sorted` (rows to gather too) the entry's position instead.
   unsigned k[kChunkItems];
+  int v[kChunkItems];
 #pragma unroll
   for (int j = 0; j < kChunkItems; ++j) {
     const int64_t p = lo + (int64_t)j * kChunkBlock + threadIdx.x;
     k[j] = p < hi ? keys[p] : ~0u;
+    v[j] = sorted ? j * kChunkBlock + (int)threadIdx.x : (p < hi ? idx[p] : 0);
   }
 #pragma unroll
   for (int j = 0; j < kChunkItems; ++j)
@@ -268,15 +272,19 @@ __global__ void __launch_bounds__(kChunkBlock) k_sort_chunks(int kind, int64_t n
   for (int j = 0; j < kChunkItems; ++j) {
     if (k[j] != ~0u) {
       const int pos = atomicAdd(&hist[k[j] & (kFine - 1)], 1);
-      staged[pos] = j * kChunkBlock + threadIdx.x;
+      staged[pos] = v[j];
     }
   }
   __syncthreads();
   const int m = (int)(hi - lo);
-  for (int pos = threadIdx.x; pos < m; pos += kChunkBlock) {
-    const int64_t src = lo + staged[pos];
-    order[lo + pos] = idx[src];
-    if (sorted) copy_row(kind, cq, src, sorted, lo + pos);
+  if (sorted) {
+    for (int pos = threadIdx.x; pos < m; pos += kChunkBlock) {
+      const int64_t src = lo + staged[pos];
+      order[lo + pos] = idx[src];
+      copy_row(kind, cq, src, sorted, lo + pos);
+    }
+  } else {
+    for (int pos = threadIdx.x; pos < m; pos += kChunkBlock) order[lo + pos] = staged[pos];
   }
 }
 

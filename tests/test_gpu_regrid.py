@@ -120,6 +120,22 @@ def test_torch_gradcheck_small():
                                     rtol=1e-7)
 
 
+def test_torch_gradcheck_2048_cells():
+    """gradcheck at the 2,000-cell scale: 2,048 source triangles, 144 hexagon-Voronoi targets,
+    F and both vertex sets.  A full numerical Jacobian would take ~10^4 evaluations, so this
+    uses fast mode: vJu for random u, v from central differences against the adjoint."""
+    import torch
+
+    sv, sf, tv, tf, F = problem(33, 12, 1.0, seed=11, n_fields=4, inset=0.0173)
+    assert len(sf) == 2048
+    tree, rg = build(sv, sf, tv, tf)
+    F_t = torch.from_numpy(F).cuda().requires_grad_(True)
+    sv_t = torch.from_numpy(sv).cuda().requires_grad_(True)
+    tv_t = rg.topology.tgt_vertices.clone().requires_grad_(True)
+    assert torch.autograd.gradcheck(lambda a, b, c: rg(a, b, c), (F_t, sv_t, tv_t), eps=1e-6, atol=1e-9,
+                                    rtol=1e-7, fast_mode=True)
+
+
 # ------------------------------------------------------------- torch restatement
 def _torch_clip_area(subj, clip):
     """sutherland_hodgman.py:84-148 + geometry_utils.py:82-95 in torch ops (branches on
