@@ -191,10 +191,25 @@ int ct_locate_boxes_write(const ct_tree* tree, const double* boxes, int64_t n,
 int ct_locate_edges_count(const ct_tree* tree, const double* segments, int64_t n, int32_t kind,
                           const int32_t* order, const double* sorted, int64_t* counts, void* stream);
 
-/* Second pass: ii, jj and xy (n_found, 2, 2) in segment order then DFS order. */
+/* Second pass: ii, jj and xy (n_found, 2, 2) in segment order then DFS order; offsets
+ * (n + 1) is the exclusive scan of the counts. */
 int ct_locate_edges_write(const ct_tree* tree, const double* segments, int64_t n, int32_t kind,
                           const int32_t* order, const double* sorted, const int64_t* offsets,
                           int64_t* ii, int64_t* jj, double* xy, void* stream);
+
+/* The same two passes with a hit record (query.py:500-544; the reference's query-twice
+ * scheme walks the tree in both passes): the count pass also keeps the first hit_slots hit
+ * cells of every segment in `hits` (device int32, hit_slots * n, in [1, 1024]), and the write
+ * pass rebuilds the rows of segments with at most hit_slots hits from those cells (same
+ * predicates, same DFS order, identical rows) instead of walking the tree again.  Both
+ * passes take the same order / sorted.  hits == NULL: ct_locate_edges_count / _write. */
+int ct_locate_edges_count_rec(const ct_tree* tree, const double* segments, int64_t n, int32_t kind,
+                              const int32_t* order, const double* sorted, int64_t* counts,
+                              int32_t* hits, int32_t hit_slots, void* stream);
+int ct_locate_edges_write_rec(const ct_tree* tree, const double* segments, int64_t n, int32_t kind,
+                              const int32_t* order, const double* sorted, const int64_t* offsets,
+                              const int32_t* hits, int32_t hit_slots, int64_t* ii, int64_t* jj,
+                              double* xy, void* stream);
 
 /* geometry_utils.py:564-574 sort_intersections_by_edge on grouped output (ii ascending, the
  * rows of segment s in [offsets[s], offsets[s+1])): writes jj and xy (n_rows, 2, 2) stably

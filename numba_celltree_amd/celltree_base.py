@@ -204,23 +204,38 @@ class CellTree2dBase:
         _native.call(fn, self.tree_ref, D.ptr(pts), n, float(tolerance), D.ptr(out), D.stream())
         return io.out(out)
 
+    # Hit cells the count pass records per segment (slot-major, so only slots in use are
+    # touched); segments with at most that many hits are written from the record instead of
+    # a second tree walk.  The long segments carry most of the rows, so the record pays off
+    # only when it covers them: up to EDGE_HIT_SLOTS slots within EDGE_HIT_BUDGET int32
+    # entries (1 GiB); below 4 slots the write pass walks the tree again.
+    EDGE_HIT_SLOTS = 256
+    EDGE_HIT_BUDGET = 1 << 28
+
+    def _edge_hit_slots(self, n: int) -> int:
+        slots = min(int(self.EDGE_HIT_SLOTS), int(self.EDGE_HIT_BUDGET) // max(n, 1))
+        return slots if slots >= 4 else 0
+
     def _locate_edges(self, edge_coords, kind: int):
-        """count -> scan -> write, then the per-segment sort by t (device tensors)."""
+        """count (recording hits) -> scan -> write, then the per-segment sort by t (device
+        tensors)."""
         n = int(edge_coords.shape[0])
         st = D.stream()
         order, srt = self._order(2, edge_coords)
         counts = D.empty(n, torch.int64)
+        slots = self._edge_hit_slots(n)
+        hits = D.empty(slots * max(n, 1), torch.int32) if slots > 0 else None  # 0: no record
         _native.call(
-            "ct_locate_edges_count", self.tree_ref, D.ptr(edge_coords), n, kind, D.ptr(order), D.ptr(srt),
-            D.ptr(counts), st
+            "ct_locate_edges_count_rec", self.tree_ref, D.ptr(edge_coords), n, kind, D.ptr(order), D.ptr(srt),
+            D.ptr(counts), D.ptr(hits), slots, st
         )
         offsets, total = D.exclusive_scan(counts)
         ii = D.empty(total, torch.int64)
         jj = D.empty(total, torch.int64)
         xy = D.empty((total, 2, 2), torch.float64)
         _native.call(
-            "ct_locate_edges_write", self.tree_ref, D.ptr(edge_coords), n, kind, D.ptr(order), D.ptr(srt),
-            D.ptr(offsets), D.ptr(ii), D.ptr(jj), D.ptr(xy), st,
+            "ct_locate_edges_write_rec", self.tree_ref, D.ptr(edge_coords), n, kind, D.ptr(order), D.ptr(srt),
+            D.ptr(offsets), D.ptr(hits), slots, D.ptr(ii), D.ptr(jj), D.ptr(xy), st,
         )
         jj_sorted = torch.empty_like(jj)
         xy_sorted = torch.empty_like(xy)

@@ -434,13 +434,19 @@ __global__ void __launch_bounds__(kBlock) k_boxes(TreeView t, const double* __re
 }
 
 // ----------------------------------------------------------------- segments
+// With a hit record (hits != null, slot-major: hit c of the lane at position i in
+// hits[c * n + i]), the count pass keeps the first `slots` hit cells of every query, and the
+// write pass replays the queries with at most `slots` hits: the same cell predicates on the
+// recorded cells, in the recorded (DFS) order, so the rows are identical.  Queries with more
+// hits walk the tree again.
 template <bool WRITE, int MAXV, int STACK, bool FACES>
 __global__ void __launch_bounds__(kBlock) k_edges(TreeView t, const double* __restrict__ segs,
                                                   QueryOrder ord, int64_t n,
                                                   int64_t* __restrict__ counts,
                                                   const int64_t* __restrict__ offsets,
                                                   int64_t* __restrict__ ii, int64_t* __restrict__ jj,
-                                                  double* __restrict__ xy) {
+                                                  double* __restrict__ xy, int32_t* __restrict__ hits,
+                                                  int slots) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int64_t q = query_index(ord.order, i);
@@ -449,9 +455,28 @@ __global__ void __launch_bounds__(kBlock) k_edges(TreeView t, const double* __re
   const P2 a = {s4.x, s4.y}, b = {s4.z, s4.w};
   CT_LANE_STACK(STACK, stack);
   if (WRITE) {
-    edge_query<true, MAXV, STACK, FACES>(t, a, b, q, offsets[q], ii, jj, xy, stack);
+    const int64_t o = offsets[q];
+    const int64_t cnt = offsets[q + 1] - o;
+    if (hits != nullptr && cnt <= slots) {
+      for (int64_t h = 0; h < cnt; ++h) {
+        const int c = hits[h * n + i];
+        P2 cc, dd;
+        if (FACES) {
+          edge_face<MAXV>(t, c, a, b, &cc, &dd);
+        } else {
+          edge_edge(t, c, a, b, &cc, &dd);
+        }
+        ii[o + h] = q;
+        jj[o + h] = c;
+        reinterpret_cast<double4*>(xy)[o + h] = make_double4(cc.x, cc.y, dd.x, dd.y);
+      }
+    } else {
+      edge_query<true, MAXV, STACK, FACES>(t, a, b, q, o, ii, jj, xy, stack);
+    }
   } else {
-    counts[q] = edge_query<false, MAXV, STACK, FACES>(t, a, b, q, 0, nullptr, nullptr, nullptr, stack);
+    HitRecord rec;
+    if (hits != nullptr) rec = HitRecord{hits + i, slots, n};
+    counts[q] = edge_query<false, MAXV, STACK, FACES>(t, a, b, q, 0, nullptr, nullptr, nullptr, stack, rec);
   }
 }
 
@@ -782,10 +807,13 @@ struct EdgesLaunch {
   int64_t* ii;
   int64_t* jj;
   double* xy;
+  int32_t* hits;
+  int slots;
   hipStream_t st;
   template <int S>
   int run() {
-    k_edges<WRITE, MAXV, S, FACES><<<grid_for(n), kBlock, 0, st>>>(t, segs, order, n, counts, offsets, ii, jj, xy);
+    k_edges<WRITE, MAXV, S, FACES><<<grid_for(n), kBlock, 0, st>>>(t, segs, order, n, counts, offsets, ii, jj, xy,
+                                                                    hits, slots);
     CT_LAUNCH_CHECK();
     return CT_OK;
   }
@@ -794,18 +822,19 @@ struct EdgesLaunch {
 template <bool WRITE>
 int launch_edges(const ct_tree* tree, const double* segs, QueryOrder order, int64_t n, int kind,
                  int64_t* counts, const int64_t* offsets, int64_t* ii, int64_t* jj, double* xy,
-                 hipStream_t st) {
+                 int32_t* hits, int slots, hipStream_t st) {
   const TreeView t = view(tree);
   const int d = tree->depth, nm = tree->n_max_vert;
   if (kind == 0)
-    return with_stack(d, EdgesLaunch<WRITE, 4, false>{t, segs, order, n, counts, offsets, ii, jj, xy, st});
+    return with_stack(d, EdgesLaunch<WRITE, 4, false>{t, segs, order, n, counts, offsets, ii, jj, xy, hits, slots, st});
   if (nm <= 4)
-    return with_stack(d, EdgesLaunch<WRITE, 4, true>{t, segs, order, n, counts, offsets, ii, jj, xy, st});
+    return with_stack(d, EdgesLaunch<WRITE, 4, true>{t, segs, order, n, counts, offsets, ii, jj, xy, hits, slots, st});
   if (nm <= 8)
-    return with_stack(d, EdgesLaunch<WRITE, 8, true>{t, segs, order, n, counts, offsets, ii, jj, xy, st});
+    return with_stack(d, EdgesLaunch<WRITE, 8, true>{t, segs, order, n, counts, offsets, ii, jj, xy, hits, slots, st});
   if (nm <= 16)
-    return with_stack(d, EdgesLaunch<WRITE, 16, true>{t, segs, order, n, counts, offsets, ii, jj, xy, st});
-  return with_stack(d, EdgesLaunch<WRITE, 32, true>{t, segs, order, n, counts, offsets, ii, jj, xy, st});
+    return with_stack(d, EdgesLaunch<WRITE, 16, true>{t, segs, order, n, counts, offsets, ii, jj, xy, hits, slots,
+                                                      st});
+  return with_stack(d, EdgesLaunch<WRITE, 32, true>{t, segs, order, n, counts, offsets, ii, jj, xy, hits, slots, st});
 }
 
 template <int MAXV, int LEAF>
@@ -955,29 +984,45 @@ int ct_locate_boxes_write(const ct_tree* tree, const double* boxes, int64_t n, c
                                                    jj, as_stream(stream)});
 }
 
-int ct_locate_edges_count(const ct_tree* tree, const double* segments, int64_t n, int32_t kind,
-                          const int32_t* order, const double* sorted, int64_t* counts, void* stream) {
+int ct_locate_edges_count_rec(const ct_tree* tree, const double* segments, int64_t n, int32_t kind,
+                              const int32_t* order, const double* sorted, int64_t* counts, int32_t* hits,
+                              int32_t hit_slots, void* stream) {
   int rc = check_tree(tree);
   if (rc) return rc;
   CT_REQUIRE(n >= 0 && (kind == 0 || kind == 1), "bad arguments");
   CT_REQUIRE(kind == 1 || tree->n_max_vert == 2, "edge-edge queries need an edge tree");
   CT_REQUIRE(sorted == nullptr || order != nullptr, "sorted queries need their order");
+  CT_REQUIRE(hits == nullptr || (hit_slots >= 1 && hit_slots <= 1024), "hit_slots must be in [1, 1024]");
   if (n == 0) return CT_OK;
   return launch_edges<false>(tree, segments, QueryOrder{order, sorted}, n, kind, counts, nullptr, nullptr,
-                             nullptr, nullptr, as_stream(stream));
+                             nullptr, nullptr, hits, hits ? hit_slots : 0, as_stream(stream));
+}
+
+int ct_locate_edges_write_rec(const ct_tree* tree, const double* segments, int64_t n, int32_t kind,
+                              const int32_t* order, const double* sorted, const int64_t* offsets,
+                              const int32_t* hits, int32_t hit_slots, int64_t* ii, int64_t* jj, double* xy,
+                              void* stream) {
+  int rc = check_tree(tree);
+  if (rc) return rc;
+  CT_REQUIRE(n >= 0 && (kind == 0 || kind == 1), "bad arguments");
+  CT_REQUIRE(kind == 1 || tree->n_max_vert == 2, "edge-edge queries need an edge tree");
+  CT_REQUIRE(sorted == nullptr || order != nullptr, "sorted queries need their order");
+  CT_REQUIRE(hits == nullptr || (hit_slots >= 1 && hit_slots <= 1024), "hit_slots must be in [1, 1024]");
+  if (n == 0) return CT_OK;
+  return launch_edges<true>(tree, segments, QueryOrder{order, sorted}, n, kind, nullptr, offsets, ii, jj, xy,
+                            const_cast<int32_t*>(hits), hits ? hit_slots : 0, as_stream(stream));
+}
+
+int ct_locate_edges_count(const ct_tree* tree, const double* segments, int64_t n, int32_t kind,
+                          const int32_t* order, const double* sorted, int64_t* counts, void* stream) {
+  return ct_locate_edges_count_rec(tree, segments, n, kind, order, sorted, counts, nullptr, 0, stream);
 }
 
 int ct_locate_edges_write(const ct_tree* tree, const double* segments, int64_t n, int32_t kind,
                           const int32_t* order, const double* sorted, const int64_t* offsets, int64_t* ii,
                           int64_t* jj, double* xy, void* stream) {
-  int rc = check_tree(tree);
-  if (rc) return rc;
-  CT_REQUIRE(n >= 0 && (kind == 0 || kind == 1), "bad arguments");
-  CT_REQUIRE(kind == 1 || tree->n_max_vert == 2, "edge-edge queries need an edge tree");
-  CT_REQUIRE(sorted == nullptr || order != nullptr, "sorted queries need their order");
-  if (n == 0) return CT_OK;
-  return launch_edges<true>(tree, segments, QueryOrder{order, sorted}, n, kind, nullptr, offsets, ii, jj, xy,
-                            as_stream(stream));
+  return ct_locate_edges_write_rec(tree, segments, n, kind, order, sorted, offsets, nullptr, 0, ii, jj, xy,
+                                   stream);
 }
 
 int ct_sort_intersections_by_edge(const double* segments, const int64_t* offsets, const int64_t* ii,

@@ -289,14 +289,24 @@ __device__ __forceinline__ bool seg_meets_box(P2 a, double dx, double dy, double
   return !(t0 > t1);
 }
 
+// The first `slots` hit cells of one query, kept by a count pass so that the write pass can
+// replay them instead of walking the tree again: hit c at cells[c * stride].
+struct HitRecord {
+  int32_t* cells = nullptr;
+  int slots = 0;
+  int64_t stride = 0;
+};
+
 // query.py:357-455 locate_edge for segment q = (a, b): Cohen-Sutherland against the tree
 // bbox, then slab pruning per node (the t-tests of :428-441 always hold once the slab test
-// passed, but are kept).  Returns the number of hits; with WRITE, hit rows go to o, o+1, ...
+// passed, but are kept).  Returns the number of hits; with WRITE, hit rows go to o, o+1, ...;
+// without, the first rec.slots hit cells go to rec (when rec.cells is set).
 // Segment-face queries on a tree with node boxes also skip subtrees by seg_meets_box.
 template <bool WRITE, int MAXV, int STACK, bool FACES>
 __device__ __forceinline__ int64_t edge_query(const TreeView& t, P2 a, P2 b, int64_t q, int64_t o,
                                               int64_t* __restrict__ ii, int64_t* __restrict__ jj,
-                                              double* __restrict__ xy, LaneStack<STACK>& stack) {
+                                              double* __restrict__ xy, LaneStack<STACK>& stack,
+                                              HitRecord rec = HitRecord{}) {
   int64_t cnt = 0;
   P2 c0, d0;
   if (!cohen_sutherland(a, b, t.bbox, &c0, &d0)) return 0;
@@ -362,6 +372,8 @@ __device__ __forceinline__ int64_t edge_query(const TreeView& t, P2 a, P2 b, int
           ii[o + cnt] = q;
           jj[o + cnt] = c;
           reinterpret_cast<double4*>(xy)[o + cnt] = make_double4(cc.x, cc.y, dd.x, dd.y);
+        } else if (rec.cells != nullptr && cnt < rec.slots) {
+          rec.cells[cnt * rec.stride] = c;
         }
         ++cnt;
       }

@@ -435,3 +435,55 @@ def test_intersect_edges_pruning_is_exact(trees):
         g._ctree.node_boxes = saved
     assert np.array_equal(ui, gi) and np.array_equal(uj, gj)
     assert np.array_equal(uxy.view(np.int64), gxy.view(np.int64))
+
+
+def test_intersect_edges_hit_record_is_exact(trees):
+    """The write pass replayed from the count pass's hit record gives the rows of a second
+    tree walk, bit for bit: record sizes 1, 3 and 16 mix replayed and re-walked segments
+    (the segments range from point-like to across the mesh)."""
+    import torch
+
+    from numba_celltree_amd import _device as D
+    from numba_celltree_amd import _native
+
+    name, v, f, g, o = trees
+    L = _extent(v)
+    lo = v.min(0)
+    rng = np.random.default_rng(21)
+    n = 6000
+    a = lo + rng.uniform(-0.05 * L, 1.05 * L, size=(n, 2))
+    length = L * np.exp(rng.uniform(np.log(1e-4), 0.0, size=(n, 1)))
+    ang = rng.uniform(0, 2 * np.pi, size=(n, 1))
+    b = a + length * np.concatenate([np.cos(ang), np.sin(ang)], axis=1)
+    seg = torch.from_numpy(np.ascontiguousarray(np.stack([a, b], axis=1))).cuda()
+    log(f"edges hit record {name} n={n}")
+    st = D.stream()
+    order, srt = g._order(2, seg)
+
+    def run(slots):
+        counts = D.empty(n, torch.int64)
+        hits = D.empty(max(slots, 1) * n, torch.int32) if slots else None
+        if slots:
+            _native.call("ct_locate_edges_count_rec", g.tree_ref, D.ptr(seg), n, 1, D.ptr(order), D.ptr(srt),
+                         D.ptr(counts), D.ptr(hits), slots, st)
+        else:
+            _native.call("ct_locate_edges_count", g.tree_ref, D.ptr(seg), n, 1, D.ptr(order), D.ptr(srt),
+                         D.ptr(counts), st)
+        offsets, total = D.exclusive_scan(counts)
+        ii = D.empty(total, torch.int64)
+        jj = D.empty(total, torch.int64)
+        xy = D.empty((total, 2, 2), torch.float64)
+        if slots:
+            _native.call("ct_locate_edges_write_rec", g.tree_ref, D.ptr(seg), n, 1, D.ptr(order), D.ptr(srt),
+                         D.ptr(offsets), D.ptr(hits), slots, D.ptr(ii), D.ptr(jj), D.ptr(xy), st)
+        else:
+            _native.call("ct_locate_edges_write", g.tree_ref, D.ptr(seg), n, 1, D.ptr(order), D.ptr(srt),
+                         D.ptr(offsets), D.ptr(ii), D.ptr(jj), D.ptr(xy), st)
+        return counts.cpu().numpy(), ii.cpu().numpy(), jj.cpu().numpy(), xy.cpu().numpy()
+
+    c0, i0, j0, x0 = run(0)
+    assert (c0 > 16).any() and (c0 <= 1).any()
+    for slots in (1, 3, 16):
+        c, i, j, x = run(slots)
+        assert np.array_equal(c, c0) and np.array_equal(i, i0) and np.array_equal(j, j0), slots
+        assert np.array_equal(x.view(np.int64), x0.view(np.int64)), slots
