@@ -144,8 +144,8 @@ def extras(args):
 
     for name, fn in (("regrid_fwd_16f_ms", fwd), ("regrid_fwd_bwd_16f_ms", fwd_bwd)):
         fn()
-        wall, _ = time_calls(fn, 5)
-        out[name] = 1e3 * wall / 5
+        wall, _ = time_calls(fn, 20)
+        out[name] = 1e3 * wall / 20
     out["regrid_fwd_bwd_over_fwd"] = out["regrid_fwd_bwd_16f_ms"] / out["regrid_fwd_16f_ms"]
     del tree, rg
     sv, sf, tv, tf, _ = S.config3()

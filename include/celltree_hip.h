@@ -186,6 +186,18 @@ int ct_locate_boxes_write(const ct_tree* tree, const double* boxes, int64_t n,
                           const int32_t* order, const double* sorted, const int64_t* offsets,
                           int64_t* ii, int64_t* jj, void* stream);
 
+/* The same two passes with a hit record (as ct_locate_edges_count_rec / _write_rec): the
+ * count pass keeps the first hit_slots hit cells of every box in `hits` (device int32,
+ * hit_slots * n, in [1, 1024]); the write pass copies them for boxes with at most hit_slots
+ * hits instead of walking the tree again.  hits == NULL: ct_locate_boxes_count / _write. */
+int ct_locate_boxes_count_rec(const ct_tree* tree, const double* boxes, int64_t n,
+                              const int32_t* order, const double* sorted, int64_t* counts,
+                              int32_t* hits, int32_t hit_slots, void* stream);
+int ct_locate_boxes_write_rec(const ct_tree* tree, const double* boxes, int64_t n,
+                              const int32_t* order, const double* sorted, const int64_t* offsets,
+                              const int32_t* hits, int32_t hit_slots, int64_t* ii, int64_t* jj,
+                              void* stream);
+
 /* query.py:500-544 locate_edge_faces (kind 1) / locate_edge_edges (kind 0), first pass;
  * `order` and `sorted` as for ct_locate_boxes_count. */
 int ct_locate_edges_count(const ct_tree* tree, const double* segments, int64_t n, int32_t kind,

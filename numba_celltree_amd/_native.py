@@ -150,6 +150,8 @@ SIGNATURES = {
     "ct_locate_boxes_write": (ctypes.c_int, [_TP, _P, _I64, _P, _P, _P, _P, _P, _P]),
     "ct_locate_edges_count": (ctypes.c_int, [_TP, _P, _I64, _I32, _P, _P, _P, _P]),
     "ct_locate_edges_write": (ctypes.c_int, [_TP, _P, _I64, _I32, _P, _P, _P, _P, _P, _P, _P]),
+    "ct_locate_boxes_count_rec": (ctypes.c_int, [_TP, _P, _I64, _P, _P, _P, _P, _I32, _P]),
+    "ct_locate_boxes_write_rec": (ctypes.c_int, [_TP, _P, _I64, _P, _P, _P, _P, _I32, _P, _P, _P]),
     "ct_locate_edges_count_rec": (ctypes.c_int, [_TP, _P, _I64, _I32, _P, _P, _P, _P, _I32, _P]),
     "ct_locate_edges_write_rec": (ctypes.c_int, [_TP, _P, _I64, _I32, _P, _P, _P, _P, _I32, _P, _P, _P, _P]),
     "ct_sort_intersections_by_edge": (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _P, _P, _P]),

@@ -91,14 +91,16 @@ class CellTree2d(CellTree2dBase):
         st = D.stream()
         order, srt = self._order(1, d_boxes)
         counts = D.empty(n, torch.int64)
-        _native.call("ct_locate_boxes_count", self.tree_ref, D.ptr(d_boxes), n, D.ptr(order), D.ptr(srt),
-                     D.ptr(counts), st)
+        slots = self._edge_hit_slots(n)  # same record budget as intersect_edges
+        hits = D.empty(slots * max(n, 1), torch.int32) if slots > 0 else None
+        _native.call("ct_locate_boxes_count_rec", self.tree_ref, D.ptr(d_boxes), n, D.ptr(order), D.ptr(srt),
+                     D.ptr(counts), D.ptr(hits), slots, st)
         offsets, total = D.exclusive_scan(counts)
         ii = D.empty(total, torch.int64)
         jj = D.empty(total, torch.int64)
         _native.call(
-            "ct_locate_boxes_write", self.tree_ref, D.ptr(d_boxes), n, D.ptr(order), D.ptr(srt), D.ptr(offsets),
-            D.ptr(ii), D.ptr(jj), st,
+            "ct_locate_boxes_write_rec", self.tree_ref, D.ptr(d_boxes), n, D.ptr(order), D.ptr(srt),
+            D.ptr(offsets), D.ptr(hits), slots, D.ptr(ii), D.ptr(jj), st,
         )
         return ii, jj
 

@@ -370,16 +370,29 @@ __global__ void __launch_bounds__(kBlock) k_pack_leaf(TreeView t, double* __rest
 
 // ----------------------------------------------------------------- boxes
 // query.py:177-244 locate_box: strict overlap with the tree bbox first; the left child is
-// pushed before the right, so the right child is visited first.
+// pushed before the right, so the right child is visited first.  With a hit record (hits !=
+// null, slot-major as in k_edges) the count pass keeps the first `slots` hit cells, and the
+// write pass copies them for boxes with at most `slots` hits instead of walking again.
 template <bool WRITE, int STACK>
 __global__ void __launch_bounds__(kBlock) k_boxes(TreeView t, const double* __restrict__ boxes,
                                                   QueryOrder ord, int64_t n,
                                                   int64_t* __restrict__ counts,
                                                   const int64_t* __restrict__ offsets,
-                                                  int64_t* __restrict__ ii, int64_t* __restrict__ jj) {
+                                                  int64_t* __restrict__ ii, int64_t* __restrict__ jj,
+                                                  int32_t* __restrict__ hits, int slots) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int64_t q = query_index(ord.order, i);
+  if (WRITE && hits != nullptr) {
+    const int64_t o = offsets[q], total = offsets[q + 1] - o;
+    if (total <= slots) {
+      for (int64_t h = 0; h < total; ++h) {
+        ii[o + h] = q;
+        jj[o + h] = hits[h * n + i];
+      }
+      return;
+    }
+  }
   const Box box = ord.sorted ? ld_box(ord.sorted, i) : ld_box(boxes, q);
   // Box queries keep deep stacks (a box often overlaps both children), which would spill
   // past LaneStack's LDS slots on most queries: a VGPR array is faster here.
@@ -422,6 +435,7 @@ __global__ void __launch_bounds__(kBlock) k_boxes(TreeView t, const double* __re
             jj[o] = c;
             ++o;
           } else {
+            if (hits != nullptr && cnt < slots) hits[cnt * n + i] = c;
             ++cnt;
           }
         }
@@ -787,10 +801,12 @@ struct BoxesLaunch {
   const int64_t* offsets;
   int64_t* ii;
   int64_t* jj;
+  int32_t* hits;
+  int slots;
   hipStream_t st;
   template <int S>
   int run() {
-    k_boxes<WRITE, S><<<grid_for(n), kBlock, 0, st>>>(t, boxes, order, n, counts, offsets, ii, jj);
+    k_boxes<WRITE, S><<<grid_for(n), kBlock, 0, st>>>(t, boxes, order, n, counts, offsets, ii, jj, hits, slots);
     CT_LAUNCH_CHECK();
     return CT_OK;
   }
@@ -959,29 +975,45 @@ int ct_locate_points_on_edge(const ct_tree* tree, const double* points, int64_t 
                                          out, as_stream(stream));
 }
 
-int ct_locate_boxes_count(const ct_tree* tree, const double* boxes, int64_t n, const int32_t* order,
-                          const double* sorted, int64_t* counts, void* stream) {
+int ct_locate_boxes_count_rec(const ct_tree* tree, const double* boxes, int64_t n, const int32_t* order,
+                              const double* sorted, int64_t* counts, int32_t* hits, int32_t hit_slots,
+                              void* stream) {
   int rc = check_tree(tree);
   if (rc) return rc;
   CT_REQUIRE(n >= 0, "n must be >= 0");
   CT_REQUIRE(sorted == nullptr || order != nullptr, "sorted queries need their order");
+  CT_REQUIRE(hits == nullptr || (hit_slots >= 1 && hit_slots <= 1024), "hit_slots must be in [1, 1024]");
   if (n == 0) return CT_OK;
   const TreeView t = view(tree);
   return with_stack(tree->depth, BoxesLaunch<false>{t, boxes, QueryOrder{order, sorted}, n, counts, nullptr,
-                                                    nullptr, nullptr, as_stream(stream)});
+                                                    nullptr, nullptr, hits, hits ? hit_slots : 0,
+                                                    as_stream(stream)});
+}
+
+int ct_locate_boxes_write_rec(const ct_tree* tree, const double* boxes, int64_t n, const int32_t* order,
+                              const double* sorted, const int64_t* offsets, const int32_t* hits,
+                              int32_t hit_slots, int64_t* ii, int64_t* jj, void* stream) {
+  int rc = check_tree(tree);
+  if (rc) return rc;
+  CT_REQUIRE(n >= 0, "n must be >= 0");
+  CT_REQUIRE(sorted == nullptr || order != nullptr, "sorted queries need their order");
+  CT_REQUIRE(hits == nullptr || (hit_slots >= 1 && hit_slots <= 1024), "hit_slots must be in [1, 1024]");
+  if (n == 0) return CT_OK;
+  const TreeView t = view(tree);
+  return with_stack(tree->depth, BoxesLaunch<true>{t, boxes, QueryOrder{order, sorted}, n, nullptr, offsets, ii,
+                                                   jj, const_cast<int32_t*>(hits), hits ? hit_slots : 0,
+                                                   as_stream(stream)});
+}
+
+int ct_locate_boxes_count(const ct_tree* tree, const double* boxes, int64_t n, const int32_t* order,
+                          const double* sorted, int64_t* counts, void* stream) {
+  return ct_locate_boxes_count_rec(tree, boxes, n, order, sorted, counts, nullptr, 0, stream);
 }
 
 int ct_locate_boxes_write(const ct_tree* tree, const double* boxes, int64_t n, const int32_t* order,
                           const double* sorted, const int64_t* offsets, int64_t* ii, int64_t* jj,
                           void* stream) {
-  int rc = check_tree(tree);
-  if (rc) return rc;
-  CT_REQUIRE(n >= 0, "n must be >= 0");
-  CT_REQUIRE(sorted == nullptr || order != nullptr, "sorted queries need their order");
-  if (n == 0) return CT_OK;
-  const TreeView t = view(tree);
-  return with_stack(tree->depth, BoxesLaunch<true>{t, boxes, QueryOrder{order, sorted}, n, nullptr, offsets, ii,
-                                                   jj, as_stream(stream)});
+  return ct_locate_boxes_write_rec(tree, boxes, n, order, sorted, offsets, nullptr, 0, ii, jj, stream);
 }
 
 int ct_locate_edges_count_rec(const ct_tree* tree, const double* segments, int64_t n, int32_t kind,

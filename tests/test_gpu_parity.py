@@ -487,3 +487,45 @@ def test_intersect_edges_hit_record_is_exact(trees):
         c, i, j, x = run(slots)
         assert np.array_equal(c, c0) and np.array_equal(i, i0) and np.array_equal(j, j0), slots
         assert np.array_equal(x.view(np.int64), x0.view(np.int64)), slots
+
+
+def test_locate_boxes_hit_record_is_exact(trees):
+    """Box pairs copied from the count pass's hit record equal a second tree walk's: record
+    sizes 1, 5 and 32 with boxes from a fraction of a cell to a third of the mesh."""
+    import torch
+
+    from numba_celltree_amd import _device as D
+    from numba_celltree_amd import _native
+
+    name, v, f, g, o = trees
+    L = _extent(v)
+    lo = v.min(0)
+    rng = np.random.default_rng(23)
+    n = 6000
+    c = lo + rng.uniform(-0.05 * L, 1.05 * L, size=(n, 2))
+    half = 0.5 * L * np.exp(rng.uniform(np.log(1e-4), np.log(0.3), size=(n, 2)))
+    boxes = np.column_stack([c[:, 0] - half[:, 0], c[:, 0] + half[:, 0], c[:, 1] - half[:, 1], c[:, 1] + half[:, 1]])
+    d_boxes = torch.from_numpy(np.ascontiguousarray(boxes)).cuda()
+    log(f"boxes hit record {name} n={n}")
+    st = D.stream()
+    order, srt = g._order(1, d_boxes)
+
+    def run(slots):
+        counts = D.empty(n, torch.int64)
+        hits = D.empty(slots * n, torch.int32) if slots else None
+        _native.call("ct_locate_boxes_count_rec", g.tree_ref, D.ptr(d_boxes), n, D.ptr(order), D.ptr(srt),
+                     D.ptr(counts), D.ptr(hits), slots, st)
+        offsets, total = D.exclusive_scan(counts)
+        ii = D.empty(total, torch.int64)
+        jj = D.empty(total, torch.int64)
+        _native.call("ct_locate_boxes_write_rec", g.tree_ref, D.ptr(d_boxes), n, D.ptr(order), D.ptr(srt),
+                     D.ptr(offsets), D.ptr(hits), slots, D.ptr(ii), D.ptr(jj), st)
+        return counts.cpu().numpy(), ii.cpu().numpy(), jj.cpu().numpy()
+
+    c0, i0, j0 = run(0)
+    assert (c0 > 32).any() and (c0 <= 1).any()
+    oi, oj = o.locate_boxes(boxes)
+    assert np.array_equal(i0, oi) and np.array_equal(j0, oj)
+    for slots in (1, 5, 32):
+        c, i, j = run(slots)
+        assert np.array_equal(c, c0) and np.array_equal(i, i0) and np.array_equal(j, j0), slots
