@@ -465,6 +465,36 @@ __global__ void __launch_bounds__(kBlock) k_apply(const int64_t* __restrict__ to
   y[i] = acc / tarea[t];
 }
 
+// k_apply_t with one lane per source and all NF fields in registers: each pair's weight and
+// target are loaded once and its gy row is read as a whole, instead of by NF lanes that each
+// repeat the index and weight loads.  Same sums, same order, so the same bits.
+template <int NF>
+__global__ void __launch_bounds__(kBlock) k_apply_t_rows(const int64_t* __restrict__ soff,
+                                                         const int64_t* __restrict__ perm,
+                                                         const int64_t* __restrict__ src_tgt,
+                                                         const double* __restrict__ area,
+                                                         const double* __restrict__ tarea,
+                                                         const double* __restrict__ gy, int64_t n_s,
+                                                         double* __restrict__ gF) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_s) return;
+  double acc[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) acc[f] = 0.0;
+  const int64_t end = soff[s + 1];
+  for (int64_t m = soff[s]; m < end; ++m) {
+    const int64_t k = perm[m];
+    const int64_t t = src_tgt[m];
+    const double w = area[k] / tarea[t];
+    const double* row = gy + t * NF;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) acc[f] += w * row[f];
+  }
+  double* out = gF + s * NF;
+#pragma unroll
+  for (int f = 0; f < NF; ++f) out[f] = acc[f];
+}
+
 // gF[s, f] = sum over the source's pairs (perm order) of area_k / area_t gy[t_k, f].
 // src_tgt[m] = tt[perm[m]] (the pairs' targets in by-source order) makes the pair index and
 // its target two independent loads instead of a chain perm -> tt -> gy.
@@ -616,8 +646,20 @@ int ct_regrid_apply_transpose(const int64_t* src_offsets, const int64_t* perm, c
                               int64_t n_src, double* gF, void* stream) {
   CT_REQUIRE(n_src >= 0 && n_fields >= 1, "bad sizes");
   if (n_src == 0) return CT_OK;
-  k_apply_t<<<grid_for(n_src * n_fields), kBlock, 0, as_stream(stream)>>>(src_offsets, perm, src_tgt, area,
-                                                                          tgt_area, gy, n_fields, n_src, gF);
+  hipStream_t st = as_stream(stream);
+  const unsigned g = grid_for(n_src);
+  switch (n_fields) {
+    case 1: k_apply_t_rows<1><<<g, kBlock, 0, st>>>(src_offsets, perm, src_tgt, area, tgt_area, gy, n_src, gF); break;
+    case 2: k_apply_t_rows<2><<<g, kBlock, 0, st>>>(src_offsets, perm, src_tgt, area, tgt_area, gy, n_src, gF); break;
+    case 4: k_apply_t_rows<4><<<g, kBlock, 0, st>>>(src_offsets, perm, src_tgt, area, tgt_area, gy, n_src, gF); break;
+    case 8: k_apply_t_rows<8><<<g, kBlock, 0, st>>>(src_offsets, perm, src_tgt, area, tgt_area, gy, n_src, gF); break;
+    case 16:
+      k_apply_t_rows<16><<<g, kBlock, 0, st>>>(src_offsets, perm, src_tgt, area, tgt_area, gy, n_src, gF);
+      break;
+    default:
+      k_apply_t<<<grid_for(n_src * n_fields), kBlock, 0, st>>>(src_offsets, perm, src_tgt, area, tgt_area, gy,
+                                                               n_fields, n_src, gF);
+  }
   CT_LAUNCH_CHECK();
   return CT_OK;
 }
