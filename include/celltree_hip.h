@@ -419,6 +419,15 @@ int ct_batch_edges_count(const ct_batch* batch, const double* segments, int64_t*
 int ct_batch_edges_write(const ct_batch* batch, const double* segments, const int64_t* offsets,
                          int64_t capacity, int64_t* ii, int64_t* jj, double* xy, void* stream);
 
+/* The two passes with a hit record (as ct_locate_edges_count_rec / _write_rec): `hits` is a
+ * device int32 array of hit_slots * n_queries (slot-major over the concatenated query index),
+ * hit_slots in [1, 1024].  hits == NULL: ct_batch_edges_count / _write. */
+int ct_batch_edges_count_rec(const ct_batch* batch, const double* segments, int64_t n_queries,
+                             int64_t* counts, int32_t* hits, int32_t hit_slots, void* stream);
+int ct_batch_edges_write_rec(const ct_batch* batch, const double* segments, int64_t n_queries,
+                             const int64_t* offsets, const int32_t* hits, int32_t hit_slots,
+                             int64_t capacity, int64_t* ii, int64_t* jj, double* xy, void* stream);
+
 /* geometry_utils.py:564-574 sort_intersections_by_edge over the rows of ct_batch_edges_write
  * (ii = concatenated segment index), like ct_sort_intersections_by_edge but with the row
  * count read on the device from offsets[n_segments]: rows sorted only if it is <= capacity.

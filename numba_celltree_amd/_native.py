@@ -176,6 +176,8 @@ SIGNATURES = {
     "ct_batch_locate_points": (ctypes.c_int, [_BP, _P, _P, _P]),
     "ct_batch_edges_count": (ctypes.c_int, [_BP, _P, _P, _P]),
     "ct_batch_edges_write": (ctypes.c_int, [_BP, _P, _P, _I64, _P, _P, _P, _P]),
+    "ct_batch_edges_count_rec": (ctypes.c_int, [_BP, _P, _I64, _P, _P, _I32, _P]),
+    "ct_batch_edges_write_rec": (ctypes.c_int, [_BP, _P, _I64, _P, _P, _I32, _I64, _P, _P, _P, _P]),
     "ct_batch_sort_intersections": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P, _I64, _I32, _P, _P, _P]),
 }
 

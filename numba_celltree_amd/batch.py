@@ -192,18 +192,31 @@ class BatchPlan:
         _check(out, (self.n_queries,), torch.int64)
         _native.call("ct_batch_locate_points", self.ref, D.ptr(points), D.ptr(out), D.stream())
 
-    def edges_count(self, segments: torch.Tensor, counts: torch.Tensor) -> None:
+    def edges_count(self, segments: torch.Tensor, counts: torch.Tensor, hits: Optional[torch.Tensor] = None,
+                    slots: int = 0) -> None:
+        """First pass; with `hits` ((slots * n_queries,) int32) it also records hit cells."""
         _check(segments, (self.n_queries, 2, 2), torch.float64)
         _check(counts, (self.n_queries,), torch.int64)
-        _native.call("ct_batch_edges_count", self.ref, D.ptr(segments), D.ptr(counts), D.stream())
+        self._check_hits(hits, slots)
+        _native.call("ct_batch_edges_count_rec", self.ref, D.ptr(segments), self.n_queries, D.ptr(counts),
+                     D.ptr(hits), int(slots) if hits is not None else 0, D.stream())
 
-    def edges_write(self, segments, offsets, capacity: int, ii, jj, xy) -> None:
+    def edges_write(self, segments, offsets, capacity: int, ii, jj, xy, hits: Optional[torch.Tensor] = None,
+                    slots: int = 0) -> None:
         _check(offsets, (self.n_queries + 1,), torch.int64)
         for t in (ii, jj):
             _check(t, (capacity,), torch.int64)
         _check(xy, (capacity, 2, 2), torch.float64)
-        _native.call("ct_batch_edges_write", self.ref, D.ptr(segments), D.ptr(offsets), int(capacity),
-                     D.ptr(ii), D.ptr(jj), D.ptr(xy), D.stream())
+        self._check_hits(hits, slots)
+        _native.call("ct_batch_edges_write_rec", self.ref, D.ptr(segments), self.n_queries, D.ptr(offsets),
+                     D.ptr(hits), int(slots) if hits is not None else 0, int(capacity), D.ptr(ii), D.ptr(jj),
+                     D.ptr(xy), D.stream())
+
+    def _check_hits(self, hits, slots: int) -> None:
+        if hits is not None:
+            if not 1 <= int(slots) <= 1024:
+                raise ValueError("slots must be in [1, 1024]")
+            _check(hits, (int(slots) * self.n_queries,), torch.int32)
 
 
 def _check(t: torch.Tensor, shape, dtype) -> None:
@@ -222,10 +235,25 @@ def concat_queries(sets: Sequence, width: int) -> Tuple[torch.Tensor, np.ndarray
     return cat, counts
 
 
-class EdgeBuffers:
-    """Device buffers of a batched segment pass with room for `capacity` rows."""
+# Hit record of the batched segment passes (as CellTree2d.EDGE_HIT_SLOTS / EDGE_HIT_BUDGET):
+# up to EDGE_HIT_SLOTS recorded cells per segment within EDGE_HIT_BUDGET int32 entries.  A
+# batch is one large call, so the budget is 8 GiB (2.8% of a 288 GB device; slot-major, so
+# only slots in use are written): 256 slots for 7.5M segments.  With 35 slots (1 GiB) the
+# long segments, which carry most rows, still walked twice and the record did not pay.
+EDGE_HIT_SLOTS = 256
+EDGE_HIT_BUDGET = 1 << 31
 
-    def __init__(self, n_queries: int, capacity: int):
+
+class EdgeBuffers:
+    """Device buffers of a batched segment pass with room for `capacity` rows, and the
+    count pass's hit record (None below 4 slots per segment)."""
+
+    def __init__(self, n_queries: int, capacity: int, record: bool = True):
+        slots = min(EDGE_HIT_SLOTS, EDGE_HIT_BUDGET // max(n_queries, 1)) if record else 0
+        self.slots = slots if slots >= 4 else 0
+        self.hits = D.empty(self.slots * n_queries, torch.int32) if self.slots and n_queries else None
+        if self.hits is None:
+            self.slots = 0
         self.capacity = int(capacity)
         self.counts = D.empty(n_queries, torch.int64)
         self.offsets = D.empty(n_queries + 1, torch.int64)
@@ -241,10 +269,10 @@ def run_edges(plan: BatchPlan, segments: torch.Tensor, buf: EdgeBuffers, sort_wo
     """count -> scan -> write -> per-segment sort, all enqueued (no host synchronization)."""
     n = plan.n_queries
     st = D.stream()
-    plan.edges_count(segments, buf.counts)
+    plan.edges_count(segments, buf.counts, buf.hits, buf.slots)
     _native.call("ct_exclusive_scan", D.ptr(buf.counts), n, D.ptr(buf.offsets), D.ptr(buf.scan_ws),
                  buf.scan_ws.numel(), st)
-    plan.edges_write(segments, buf.offsets, buf.capacity, buf.ii, buf.jj, buf.xy)
+    plan.edges_write(segments, buf.offsets, buf.capacity, buf.ii, buf.jj, buf.xy, buf.hits, buf.slots)
     _native.call("ct_batch_sort_intersections", D.ptr(segments), n, D.ptr(buf.offsets), D.ptr(buf.ii),
                  D.ptr(buf.jj), D.ptr(buf.xy), buf.capacity, int(sort_workgroups), D.ptr(buf.jj_sorted),
                  D.ptr(buf.xy_sorted), st)
@@ -297,7 +325,7 @@ class BatchGraph:
         self.buf = None
         if self.eplan is not None:
             if capacity is None:
-                probe = EdgeBuffers(self.eplan.n_queries, 0)
+                probe = EdgeBuffers(self.eplan.n_queries, 0, record=False)
                 self.eplan.edges_count(segments, probe.counts)
                 _native.call("ct_exclusive_scan", D.ptr(probe.counts), self.eplan.n_queries, D.ptr(probe.offsets),
                              D.ptr(probe.scan_ws), probe.scan_ws.numel(), D.stream())
@@ -362,7 +390,7 @@ def intersect_edges(batch: TreeBatch, segment_sets: Sequence, config: BatchConfi
     segs = segs.reshape(-1, 2, 2)
     plan = batch.plan(counts, config)
     n = plan.n_queries
-    probe = EdgeBuffers(n, 0)
+    probe = EdgeBuffers(n, 0, record=False)
     plan.edges_count(segs, probe.counts)
     _native.call("ct_exclusive_scan", D.ptr(probe.counts), n, D.ptr(probe.offsets), D.ptr(probe.scan_ws),
                  probe.scan_ws.numel(), D.stream())

@@ -101,11 +101,16 @@ __global__ void __launch_bounds__(kBatchMaxBlock) k_batch_points(BatchArgs a, co
 
 // Segment-face queries.  WRITE false: counts[q]; WRITE true: rows at offsets[q] when the
 // query's rows end within `capacity` (ii = concatenated query index, jj = the tree's cell).
+// With a hit record (hits != null, slot-major over the concatenated query index: hit c of
+// query q at hits[c * n_queries + q]) the count pass keeps the first `slots` hit cells and
+// the write pass rebuilds the rows of queries with at most `slots` hits from them (the same
+// cell predicates in the same DFS order) instead of walking the tree again.
 template <bool WRITE, int MAXV, int STACK>
 __global__ void __launch_bounds__(kBatchMaxBlock) k_batch_edges(BatchArgs a, const double* __restrict__ segs, int64_t* __restrict__ counts,
                               const int64_t* __restrict__ offsets, int64_t capacity,
                               int64_t* __restrict__ ii, int64_t* __restrict__ jj,
-                              double* __restrict__ xy) {
+                              double* __restrict__ xy, int32_t* __restrict__ hits, int slots,
+                              int64_t n_queries) {
   CT_LANE_STACK(STACK, stack);
   int queue = blockIdx.x % a.n_queues, misses = 0;
   const int lane = threadIdx.x & (kWave - 1);
@@ -119,9 +124,25 @@ __global__ void __launch_bounds__(kBatchMaxBlock) k_batch_edges(BatchArgs a, con
       const double4 s4 = reinterpret_cast<const double4*>(segs)[q];
       const P2 pa = {s4.x, s4.y}, pb = {s4.z, s4.w};
       if (WRITE) {
-        if (offsets[q + 1] <= capacity) edge_query<true, MAXV, STACK, true>(t, pa, pb, q, offsets[q], ii, jj, xy, stack);
+        const int64_t o = offsets[q], end = offsets[q + 1];
+        if (end <= capacity) {
+          if (hits != nullptr && end - o <= slots) {
+            for (int64_t h = 0; h < end - o; ++h) {
+              const int c = hits[h * n_queries + q];
+              P2 cc, dd;
+              edge_face<MAXV>(t, c, pa, pb, &cc, &dd);
+              ii[o + h] = q;
+              jj[o + h] = c;
+              reinterpret_cast<double4*>(xy)[o + h] = make_double4(cc.x, cc.y, dd.x, dd.y);
+            }
+          } else {
+            edge_query<true, MAXV, STACK, true>(t, pa, pb, q, o, ii, jj, xy, stack);
+          }
+        }
       } else {
-        counts[q] = edge_query<false, MAXV, STACK, true>(t, pa, pb, q, 0, nullptr, nullptr, nullptr, stack);
+        HitRecord rec;
+        if (hits != nullptr) rec = HitRecord{hits + q, slots, n_queries};
+        counts[q] = edge_query<false, MAXV, STACK, true>(t, pa, pb, q, 0, nullptr, nullptr, nullptr, stack, rec);
       }
     }
   }
@@ -167,11 +188,15 @@ struct EdgesL {
   int64_t* ii;
   int64_t* jj;
   double* xy;
+  int32_t* hits;
+  int slots;
+  int64_t n_queries;
   unsigned grid, block;
   hipStream_t st;
   template <int MAXV, int S>
   int go() {
-    k_batch_edges<WRITE, MAXV, S><<<grid, block, 0, st>>>(a, segs, counts, offsets, capacity, ii, jj, xy);
+    k_batch_edges<WRITE, MAXV, S><<<grid, block, 0, st>>>(a, segs, counts, offsets, capacity, ii, jj, xy, hits, slots,
+                                                           n_queries);
     CT_LAUNCH_CHECK();
     return CT_OK;
   }
@@ -240,34 +265,51 @@ int ct_batch_locate_points(const ct_batch* batch, const double* points, int64_t*
                   PointsL{a, points, out, (unsigned)batch->n_workgroups, (unsigned)batch->block, st});
 }
 
-int ct_batch_edges_count(const ct_batch* batch, const double* segments, int64_t* counts, void* stream) {
+int ct_batch_edges_count_rec(const ct_batch* batch, const double* segments, int64_t n_queries, int64_t* counts,
+                             int32_t* hits, int32_t hit_slots, void* stream) {
   int rc = check_batch(batch);
   if (rc) return rc;
   CT_REQUIRE(segments && counts, "bad arguments");
+  CT_REQUIRE(hits == nullptr || (hit_slots >= 1 && hit_slots <= 1024 && n_queries >= 0),
+             "hit_slots must be in [1, 1024]");
   if (batch->n_tiles == 0) return CT_OK;
   hipStream_t st = as_stream(stream);
   rc = fill<int32_t>(batch->counters, batch->n_queues, 0, st);
   if (rc) return rc;
   const BatchArgs a = args(batch);
   return dispatch(batch->max_n_vert, batch->stack,
-                  EdgesL<false>{a, segments, counts, nullptr, 0, nullptr, nullptr, nullptr,
-                                (unsigned)batch->n_workgroups, (unsigned)batch->block, st});
+                  EdgesL<false>{a, segments, counts, nullptr, 0, nullptr, nullptr, nullptr, hits,
+                                hits ? hit_slots : 0, n_queries, (unsigned)batch->n_workgroups,
+                                (unsigned)batch->block, st});
 }
 
-int ct_batch_edges_write(const ct_batch* batch, const double* segments, const int64_t* offsets,
-                         int64_t capacity, int64_t* ii, int64_t* jj, double* xy, void* stream) {
+int ct_batch_edges_write_rec(const ct_batch* batch, const double* segments, int64_t n_queries,
+                             const int64_t* offsets, const int32_t* hits, int32_t hit_slots, int64_t capacity,
+                             int64_t* ii, int64_t* jj, double* xy, void* stream) {
   int rc = check_batch(batch);
   if (rc) return rc;
   CT_REQUIRE(segments && offsets && capacity >= 0, "bad arguments");
   CT_REQUIRE(capacity == 0 || (ii && jj && xy), "null output arrays");
+  CT_REQUIRE(hits == nullptr || (hit_slots >= 1 && hit_slots <= 1024 && n_queries >= 0),
+             "hit_slots must be in [1, 1024]");
   if (batch->n_tiles == 0) return CT_OK;
   hipStream_t st = as_stream(stream);
   rc = fill<int32_t>(batch->counters, batch->n_queues, 0, st);
   if (rc) return rc;
   const BatchArgs a = args(batch);
   return dispatch(batch->max_n_vert, batch->stack,
-                  EdgesL<true>{a, segments, nullptr, offsets, capacity, ii, jj, xy,
-                               (unsigned)batch->n_workgroups, (unsigned)batch->block, st});
+                  EdgesL<true>{a, segments, nullptr, offsets, capacity, ii, jj, xy, const_cast<int32_t*>(hits),
+                               hits ? hit_slots : 0, n_queries, (unsigned)batch->n_workgroups,
+                               (unsigned)batch->block, st});
+}
+
+int ct_batch_edges_count(const ct_batch* batch, const double* segments, int64_t* counts, void* stream) {
+  return ct_batch_edges_count_rec(batch, segments, 0, counts, nullptr, 0, stream);
+}
+
+int ct_batch_edges_write(const ct_batch* batch, const double* segments, const int64_t* offsets,
+                         int64_t capacity, int64_t* ii, int64_t* jj, double* xy, void* stream) {
+  return ct_batch_edges_write_rec(batch, segments, 0, offsets, nullptr, 0, capacity, ii, jj, xy, stream);
 }
 
 int ct_batch_sort_intersections(const double* segments, int64_t n_segments, const int64_t* offsets,

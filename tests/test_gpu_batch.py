@@ -82,6 +82,27 @@ def test_batch_intersect_edges(meshes, cfg):
         assert torch.equal(gxy.view(torch.int64), wxy.view(torch.int64)), m
 
 
+@pytest.mark.parametrize("slots", [4, 0])
+def test_batch_edges_hit_record_sizes(meshes, monkeypatch, slots):
+    """A small hit record (4 slots: most long segments walk again) and none at all give the
+    same rows as the default record."""
+    import torch
+
+    from numba_celltree_amd import batch as B
+
+    sets, trees = meshes
+    tb = B.TreeBatch(trees)
+    segs = [s for _, _, _, s in sets]
+    want = B.intersect_edges(tb, segs)
+    monkeypatch.setattr(B, "EDGE_HIT_SLOTS", slots)
+    log(f"edges hit record slots={slots}")
+    got = B.intersect_edges(tb, segs)
+    for m in range(len(sets)):
+        for a, b in zip(got[m], want[m]):
+            assert torch.equal(a.view(torch.int64) if a.is_floating_point() else a,
+                               b.view(torch.int64) if b.is_floating_point() else b), m
+
+
 def test_batch_matches_oracle(meshes):
     from numba_celltree_amd import batch as B
 
