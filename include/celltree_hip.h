@@ -63,6 +63,10 @@ typedef struct ct_tree {
                                 its last vertex; see ct_pack_leaf_coords */
   const double* node_boxes;  /* optional (may be NULL): (n_nodes + 1, 4), row j + 1 = the
                                 tight box of node j's cells; see ct_node_boxes */
+  const int32_t* start_grid; /* optional (may be NULL): (start_grid_n, start_grid_n) start
+                                nodes of point walks over bbox; see ct_start_grid */
+  int32_t start_grid_n;
+  int32_t reserved;          /* 0 */
 } ct_tree;
 
 const char* ct_last_error(void);
@@ -138,6 +142,18 @@ int ct_pack_leaf_coords(const ct_tree* tree, double* out, void* stream);
  * the same with or without it.  scratch: device int32 array of n_nodes.  Enqueues
  * 1 + depth kernels. */
 int ct_node_boxes(const ct_tree* tree, int32_t* scratch, double* out, void* stream);
+
+/* Layout helper, no reference counterpart: out (n, n) int32, row-major (grid row = y), gets
+ * for every cell of an n x n grid over the tree bbox (cell (gx, gy) spans
+ * [xmin + gx * w, xmin + (gx + 1) * w] with w = (xmax - xmin) / n, the same in y, and the
+ * border cells reach to infinity outwards) the deepest node that every point walk through
+ * that cell reaches with no deferred sibling: from the root, while the cell's span along the
+ * node's split axis is on one side of the split for all its points (v <= Lmax and v < Rmin
+ * throughout: left; v > Lmax and v >= Rmin throughout: right).  Point queries
+ * (ct_locate_points*, ct_compute_barycentric_weights, ct_locate_points_on_edge) of a tree
+ * with start_grid set begin each walk at its cell's node; their results are the same with
+ * or without it.  Requires 1 <= n <= 4096. */
+int ct_start_grid(const ct_tree* tree, int32_t n, int32_t* out, void* stream);
 
 size_t ct_locate_workspace_bytes(int64_t n);
 

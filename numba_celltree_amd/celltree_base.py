@@ -19,6 +19,22 @@ from numba_celltree_amd import _native
 from numba_celltree_amd.constants import MIN_TOLERANCE, TOLERANCE_FACTOR, CellTreeData
 
 
+# Side of the start-node grid of a tree (ct_start_grid): the power of two at or above
+# sqrt(n_nodes), at most START_GRID_MAX; trees of fewer than 64 nodes get none.  An int
+# overrides the rule (0: no grid); results are the same either way.
+START_GRID_N = None
+START_GRID_MAX = 2048
+
+
+def start_grid_size(n_nodes: int) -> int:
+    if START_GRID_N is not None:
+        return int(START_GRID_N)
+    if n_nodes < 64:
+        return 0
+    g = 1 << int(np.ceil(np.log2(np.sqrt(n_nodes))))
+    return min(g, START_GRID_MAX)
+
+
 def default_tolerance(bb_diagonal) -> float:  # celltree_base.py:51-52
     return max(MIN_TOLERANCE, TOLERANCE_FACTOR * max(bb_diagonal))
 
@@ -97,6 +113,16 @@ class CellTree2dBase:
             self.d_leaf_coords = D.empty((n, nmax, 2), torch.float64)
             _native.call("ct_pack_leaf_coords", self.tree_ref, D.ptr(self.d_leaf_coords), st)
             t.leaf_coords = self.d_leaf_coords.data_ptr()
+        t.start_grid = None
+        t.start_grid_n = 0
+        self.d_start_grid = None
+        g = start_grid_size(self.n_nodes)
+        if g:
+            # Per grid cell, the node where every point walk through it starts.
+            self.d_start_grid = D.empty((g, g), torch.int32)
+            _native.call("ct_start_grid", self.tree_ref, g, D.ptr(self.d_start_grid), st)
+            t.start_grid = self.d_start_grid.data_ptr()
+            t.start_grid_n = g
         t.node_boxes = None
         self.d_node_boxes = None
         if nmax >= 3:

@@ -353,6 +353,27 @@ int query_order(int kind, const double* q, int64_t n, const double* bbox, const 
   return CT_OK;
 }
 
+// Start-node grid (see ct_start_grid): one lane per grid cell.
+__global__ void __launch_bounds__(kBlock) k_start_grid(TreeView t, int n, int32_t* __restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (int64_t)n * n) return;
+  const int gx = (int)(k % n), gy = (int)(k / n);
+  double x0, x1, y0, y1;
+  grid_span(t.bbox.xmin, t.bbox.xmax, n, gx, &x0, &x1);
+  grid_span(t.bbox.ymin, t.bbox.ymax, n, gy, &y0, &y1);
+  int node = 0;
+  while (true) {
+    const NodeLd nd = ld_node(t.nodes, node);
+    if (nd.child == -1) break;
+    const double lo = nd.dim ? y0 : x0;
+    const double hi = nd.dim ? y1 : x1;
+    if (hi <= nd.Lmax && hi < nd.Rmin) node = nd.child;               // only the left child, always
+    else if (lo > nd.Lmax && lo >= nd.Rmin) node = nd.child + 1;      // only the right child, always
+    else break;
+  }
+  out[k] = node;
+}
+
 // Packed leaf coordinates (see ct_pack_leaf_coords).
 __global__ void __launch_bounds__(kBlock) k_pack_leaf(TreeView t, double* __restrict__ out) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -905,6 +926,17 @@ int ct_pack_leaf_coords(const ct_tree* tree, double* out, void* stream) {
   if (rc) return rc;
   CT_REQUIRE(tree->n_max_vert >= 3 && tree->n_max_vert <= 8 && out, "n_max_vert must be in [3, 8]");
   k_pack_leaf<<<grid_for(tree->n_elements), kBlock, 0, as_stream(stream)>>>(view(tree), out);
+  CT_LAUNCH_CHECK();
+  return CT_OK;
+}
+
+int ct_start_grid(const ct_tree* tree, int32_t n, int32_t* out, void* stream) {
+  int rc = check_tree(tree);
+  if (rc) return rc;
+  CT_REQUIRE(n >= 1 && n <= 4096 && out != nullptr, "grid size must be in [1, 4096]");
+  ct_tree t = *tree;
+  t.start_grid = nullptr;
+  k_start_grid<<<grid_for((int64_t)n * n), kBlock, 0, as_stream(stream)>>>(view(t), n, out);
   CT_LAUNCH_CHECK();
   return CT_OK;
 }

@@ -26,11 +26,16 @@ struct TreeView {
   int nmax;
   int n_elements;
   Box bbox;
+  const int32_t* grid;  // start-node grid (grid_n x grid_n over bbox), or null
+  int grid_n;
+  double gsx, gsy;      // grid_n / bbox extent, per axis
 };
 
 __host__ __device__ __forceinline__ TreeView view(const ct_tree& t) {
+  const int g = t.start_grid ? t.start_grid_n : 0;
   return TreeView{t.vertices, t.elements, t.bb_coords, t.nodes, t.bb_indices, t.leaf_coords, t.node_boxes,
-                  t.n_max_vert, (int)t.n_elements, Box{t.bbox[0], t.bbox[1], t.bbox[2], t.bbox[3]}};
+                  t.n_max_vert, (int)t.n_elements, Box{t.bbox[0], t.bbox[1], t.bbox[2], t.bbox[3]},
+                  t.start_grid, g, g / (t.bbox[1] - t.bbox[0]), g / (t.bbox[3] - t.bbox[2])};
 }
 
 inline TreeView view(const ct_tree* t) { return view(*t); }
@@ -188,10 +193,42 @@ __device__ __forceinline__ bool leaf_hit(const TreeView& t, P2 p, double tol, co
   return false;
 }
 
+// ----------------------------------------------------------------- start-node grid
+// An optional grid_n x grid_n grid over the tree bbox (ct_start_grid) holds, per grid cell,
+// the deepest node that every point walk starting in that cell reaches with nothing
+// deferred: from the root, while the cell's whole span along the split axis lies on one
+// side of the split (only the left child viable, or only the right), the walk's choice is
+// forced.  A walk that starts at that node visits exactly the nodes the walk from the root
+// visits after it, in the same order, so its answer is the same.  The cell's span is
+// recomputed here by the same arithmetic as in the grid build, and a point outside it (a
+// rounding at a cell border, a NaN) starts at the root.
+__host__ __device__ __forceinline__ void grid_span(double lo, double hi, int n, int g, double* a, double* b) {
+  const double w = (hi - lo) / n;
+  *a = g == 0 ? -__builtin_inf() : lo + g * w;
+  *b = g == n - 1 ? __builtin_inf() : lo + (g + 1) * w;
+}
+
+__device__ __forceinline__ int grid_index(double v, double lo, double scale, int n) {
+  double g = (v - lo) * scale;
+  g = g > 0.0 ? (g < n - 1.0 ? g : n - 1.0) : 0.0;  // NaN -> 0
+  return (int)g;
+}
+
+__device__ __forceinline__ int start_node(const TreeView& t, P2 p) {
+  if (t.grid == nullptr) return 0;
+  const int gx = grid_index(p.x, t.bbox.xmin, t.gsx, t.grid_n);
+  const int gy = grid_index(p.y, t.bbox.ymin, t.gsy, t.grid_n);
+  double x0, x1, y0, y1;
+  grid_span(t.bbox.xmin, t.bbox.xmax, t.grid_n, gx, &x0, &x1);
+  grid_span(t.bbox.ymin, t.bbox.ymax, t.grid_n, gy, &y0, &y1);
+  const bool inside = p.x >= x0 && p.x <= x1 && p.y >= y0 && p.y <= y1;
+  return inside ? t.grid[(int64_t)gy * t.grid_n + gx] : 0;
+}
+
 template <int MAXV, int STACK, bool ON_EDGE, int LEAF>
 __device__ __forceinline__ int locate_one(const TreeView& t, P2 p, double tol, LaneStack<STACK>& stack) {
   int sp = 0;
-  int node = 0;
+  int node = start_node(t, p);
   while (true) {
     NodeLd nd = ld_node(t.nodes, node);
     bool exhausted = false;

@@ -140,6 +140,137 @@ def test_locate_points_every_path(trees):
         g._ctree.leaf_coords = saved
 
 
+def _grid_points(bbox, n, rng):
+    """Points on the start grid's cell borders (the kernel's own arithmetic: lo + k * w with
+    w = (hi - lo) / n), on the bbox, outside it, and NaN."""
+    xmin, xmax, ymin, ymax = bbox
+    wx, wy = (xmax - xmin) / n, (ymax - ymin) / n
+    kx = rng.integers(0, n + 1, 4000)
+    ky = rng.integers(0, n + 1, 4000)
+    bx = xmin + kx * wx
+    by = ymin + ky * wy
+    free_x = rng.uniform(xmin, xmax, 4000)
+    free_y = rng.uniform(ymin, ymax, 4000)
+    pts = [np.stack([bx, free_y], 1), np.stack([free_x, by], 1), np.stack([bx, by], 1),
+           np.array([[xmin, ymin], [xmax, ymax], [xmin - 1.0, ymin], [np.nan, ymin], [xmin, np.nan],
+                     [np.inf, ymin], [-np.inf, -np.inf]])]
+    return np.concatenate(pts)
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 3, 64, 4096])
+def test_locate_points_start_grid(trees, n):
+    """Point walks that begin at the start-grid node give the oracle's answers for every grid
+    size (none, one cell, fine), including points on grid-cell borders, NaN and inf."""
+    import torch
+
+    from numba_celltree_amd import _device as D
+    from numba_celltree_amd import _native
+
+    name, v, f, g, o = trees
+    rng = np.random.default_rng(11 + n)
+    pts = np.concatenate([rng.uniform(v.min(0) - 1.0, v.max(0) + 1.0, size=(50_000, 2)),
+                          _grid_points(g._bbox, max(n, 1), rng)])
+    want = o.locate_points(pts)
+    saved = (g._ctree.start_grid, g._ctree.start_grid_n)
+    grid = D.empty((max(n, 1), max(n, 1)), torch.int32)
+    try:
+        if n:
+            _native.call("ct_start_grid", g.tree_ref, n, D.ptr(grid), D.stream())
+            g._ctree.start_grid, g._ctree.start_grid_n = grid.data_ptr(), n
+        else:
+            g._ctree.start_grid, g._ctree.start_grid_n = None, 0
+        log(f"start grid {name} n={n}")
+        assert np.array_equal(g.locate_points(pts), want)
+        d_pts = torch.from_numpy(pts).cuda()
+        out = D.empty(len(pts), torch.int64)
+        _native.call("ct_locate_points", g.tree_ref, D.ptr(d_pts), len(pts), g._default_tolerance(),
+                     D.ptr(out), D.stream())
+        assert np.array_equal(out.cpu().numpy(), want)
+    finally:
+        g._ctree.start_grid, g._ctree.start_grid_n = saved
+
+
+def test_start_grid_nodes_are_forced():
+    """Every start-grid node is on the root path of its cell's points with nothing deferred:
+    for sampled points of each cell, the oracle-side forced descent passes through it."""
+    from numba_celltree_amd import CellTree2d
+
+    v, f = S.config1()[:2]
+    g = CellTree2d(v, f, -1)
+    n = 16
+    grid = g.d_start_grid
+    if grid is None or grid.shape[0] != n:
+        import torch
+
+        from numba_celltree_amd import _device as D
+        from numba_celltree_amd import _native
+
+        grid = D.empty((n, n), torch.int32)
+        _native.call("ct_start_grid", g.tree_ref, n, D.ptr(grid), D.stream())
+    grid = grid.cpu().numpy()
+    nodes = g.nodes
+    xmin, xmax, ymin, ymax = g._bbox
+    rng = np.random.default_rng(5)
+    deep = 0
+    for gy in range(n):
+        for gx in range(n):
+            x0, x1 = xmin + gx * (xmax - xmin) / n, xmin + (gx + 1) * (xmax - xmin) / n
+            y0, y1 = ymin + gy * (ymax - ymin) / n, ymin + (gy + 1) * (ymax - ymin) / n
+            start = int(grid[gy, gx])
+            deep += start != 0
+            for x, y in rng.uniform([x0, y0], [x1, y1], size=(20, 2)):
+                node = 0
+                while node != start:
+                    nd = nodes[node]
+                    assert nd["child"] != -1, (gx, gy)
+                    val = y if nd["dim"] else x
+                    left, right = val <= nd["Lmax"], val >= nd["Rmin"]
+                    assert left != right, (gx, gy, node)  # one child only: nothing deferred
+                    node = int(nd["child"]) + (0 if left else 1)
+    assert deep > n * n // 2  # the grid does skip levels
+
+
+def _first_descent_deferred(nodes, x, y):
+    """Far children deferred on the near-first descent from the root to the first leaf."""
+    node, deferred = 0, 0
+    while nodes[node]["child"] != -1:
+        nd = nodes[node]
+        val = y if nd["dim"] else x
+        left, right = val <= nd["Lmax"], val >= nd["Rmin"]
+        if not (left or right):
+            break
+        if left and right:
+            deferred += 1
+            go_right = (nd["Lmax"] - val) < (val - nd["Rmin"])
+        else:
+            go_right = right
+        node = int(nd["child"]) + (1 if go_right else 0)
+    return deferred
+
+
+def test_locate_points_overlapping_cells():
+    """Heavily overlapping cells: many two-way splits per walk, so the walk stack grows past
+    its 4 LDS slots into the private spill array and back."""
+    from numba_celltree_amd import CellTree2d
+
+    rng = np.random.default_rng(3)
+    n = 4000
+    c = rng.uniform(0.0, 100.0, size=(n, 2))
+    r = rng.uniform(1.0, 30.0, size=n)
+    ang = rng.uniform(0.0, 2 * np.pi, size=(n, 1)) + np.array([0.0, 2.1, 4.2])
+    v = np.stack([c[:, :1] + r[:, None] * np.cos(ang), c[:, 1:] + r[:, None] * np.sin(ang)], -1).reshape(-1, 2)
+    f = np.arange(3 * n).reshape(n, 3)
+    g = CellTree2d(v, f, -1)
+    o = R.OracleCellTree2d(v, f, -1)
+    pts = rng.uniform(-10.0, 110.0, size=(100_000, 2))
+    # deferred siblings along the first root-to-leaf descent of a sample of walks: past the
+    # 4 LDS slots of the walk stack
+    deepest = max(_first_descent_deferred(g.nodes, x, y) for x, y in pts[:300])
+    log(f"overlapping cells: depth {g.depth}, deepest first-descent stack {deepest}")
+    assert deepest > 4
+    assert np.array_equal(g.locate_points(pts), o.locate_points(pts))
+
+
 def _degenerate_mesh(m, width, seed):
     """Jittered m x m quad grid whose cells are written in degenerate ways: plain quads,
     fill-padded triangle pairs, and rows with repeated vertices (at the start, middle and
