@@ -67,6 +67,8 @@ typedef struct ct_tree {
                                 nodes of point walks over bbox; see ct_start_grid */
   int32_t start_grid_n;
   int32_t reserved;          /* 0 */
+  const double* leaf_bb;     /* optional (may be NULL): (n_elements, 4), row k = the bbox of
+                                cell bb_indices[k]; see ct_pack_leaf_bboxes */
 } ct_tree;
 
 const char* ct_last_error(void);
@@ -134,6 +136,12 @@ int ct_locate_points(const ct_tree* tree, const double* points, int64_t n, doubl
  * a fixed n_max_vert-edge loop (a repeated vertex adds a zero-length edge, which changes no
  * point-in-polygon answer).  Requires 3 <= n_max_vert <= 8. */
 int ct_pack_leaf_coords(const ct_tree* tree, double* out, void* stream);
+
+/* Layout helper, no reference counterpart: out (n_elements, 4) gets, at row k, the bbox
+ * (xmin, xmax, ymin, ymax) of cell bb_indices[k], so box and segment queries of a tree with
+ * leaf_bb set read a leaf's cell boxes contiguously instead of through bb_indices; their
+ * results are the same with or without it. */
+int ct_pack_leaf_bboxes(const ct_tree* tree, double* out, void* stream);
 
 /* Layout helper, no reference counterpart: out (n_nodes + 1, 4) gets, at row j + 1, the
  * tight box (xmin, xmax, ymin, ymax) of node j's cells bb_indices[ptr .. ptr + size); any

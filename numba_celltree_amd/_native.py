@@ -54,6 +54,7 @@ class CtTree(ctypes.Structure):
         ("start_grid", ctypes.c_void_p),
         ("start_grid_n", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
+        ("leaf_bb", ctypes.c_void_p),
     ]
 
 
@@ -111,7 +112,7 @@ class CtRegridTopology(ctypes.Structure):
 
 
 assert ctypes.sizeof(CtNode) == 32
-assert ctypes.sizeof(CtTree) == 136
+assert ctypes.sizeof(CtTree) == 144
 assert ctypes.sizeof(CtBatchTile) == 16
 
 _P = ctypes.c_void_p
@@ -144,6 +145,7 @@ SIGNATURES = {
     "ct_pack_leaf_coords": (ctypes.c_int, [_TP, _P, _P]),
     "ct_node_boxes": (ctypes.c_int, [_TP, _P, _P, _P]),
     "ct_start_grid": (ctypes.c_int, [_TP, _I32, _P, _P]),
+    "ct_pack_leaf_bboxes": (ctypes.c_int, [_TP, _P, _P]),
     "ct_locate_workspace_bytes": (_SZ, [_I64]),
     "ct_locate_points_ordered": (ctypes.c_int, [_TP, _P, _I64, _F64, _P, _SZ, _P, _P]),
     "ct_locate_points_in_order": (ctypes.c_int, [_TP, _P, _I64, _P, _P, _F64, _P, _P]),

@@ -91,7 +91,7 @@ class CellTree2d(CellTree2dBase):
         st = D.stream()
         order, srt = self._order(1, d_boxes)
         counts = D.empty(n, torch.int64)
-        slots = self._edge_hit_slots(n)  # same record budget as intersect_edges
+        slots = self._box_hit_slots(n)
         hits = D.empty(slots * max(n, 1), torch.int32) if slots > 0 else None
         _native.call("ct_locate_boxes_count_rec", self.tree_ref, D.ptr(d_boxes), n, D.ptr(order), D.ptr(srt),
                      D.ptr(counts), D.ptr(hits), slots, st)

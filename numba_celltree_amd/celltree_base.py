@@ -23,6 +23,9 @@ from numba_celltree_amd.constants import MIN_TOLERANCE, TOLERANCE_FACTOR, CellTr
 # sqrt(n_nodes), at most START_GRID_MAX; trees of fewer than 64 nodes get none.  An int
 # overrides the rule (0: no grid); results are the same either way.
 START_GRID_N = None
+# Every tree also keeps its cell boxes in bb_indices order (ct_pack_leaf_bboxes, 32 bytes per
+# cell) for box and segment queries; False builds trees without them (same results).
+LEAF_BBOXES = True
 START_GRID_MAX = 2048
 
 
@@ -113,6 +116,13 @@ class CellTree2dBase:
             self.d_leaf_coords = D.empty((n, nmax, 2), torch.float64)
             _native.call("ct_pack_leaf_coords", self.tree_ref, D.ptr(self.d_leaf_coords), st)
             t.leaf_coords = self.d_leaf_coords.data_ptr()
+        t.leaf_bb = None
+        self.d_leaf_bb = None
+        if LEAF_BBOXES and n > 0:
+            # Leaf-ordered cell boxes: box and segment queries read a leaf's boxes directly.
+            self.d_leaf_bb = D.empty((n, 4), torch.float64)
+            _native.call("ct_pack_leaf_bboxes", self.tree_ref, D.ptr(self.d_leaf_bb), st)
+            t.leaf_bb = self.d_leaf_bb.data_ptr()
         t.start_grid = None
         t.start_grid_n = 0
         self.d_start_grid = None
@@ -238,8 +248,18 @@ class CellTree2dBase:
     EDGE_HIT_SLOTS = 256
     EDGE_HIT_BUDGET = 1 << 28
 
+    # Box queries: a box's hits are its output rows, and wide boxes have hundreds (cfg2: up to
+    # ~900), so the record is larger: up to BOX_HIT_SLOTS slots within BOX_HIT_BUDGET int32
+    # entries (4 GiB).
+    BOX_HIT_SLOTS = 1024
+    BOX_HIT_BUDGET = 1 << 30
+
     def _edge_hit_slots(self, n: int) -> int:
         slots = min(int(self.EDGE_HIT_SLOTS), int(self.EDGE_HIT_BUDGET) // max(n, 1))
+        return slots if slots >= 4 else 0
+
+    def _box_hit_slots(self, n: int) -> int:
+        slots = min(int(self.BOX_HIT_SLOTS), int(self.BOX_HIT_BUDGET) // max(n, 1))
         return slots if slots >= 4 else 0
 
     def _locate_edges(self, edge_coords, kind: int):

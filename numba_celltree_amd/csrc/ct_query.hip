@@ -374,6 +374,14 @@ __global__ void __launch_bounds__(kBlock) k_start_grid(TreeView t, int n, int32_
   out[k] = node;
 }
 
+// Leaf-ordered cell bboxes (see ct_pack_leaf_bboxes).
+__global__ void __launch_bounds__(kBlock) k_pack_leaf_bb(TreeView t, double* __restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= t.n_elements) return;
+  const int c = t.bb_indices[k];
+  reinterpret_cast<double4*>(out)[k] = reinterpret_cast<const double4*>(t.bb)[c];
+}
+
 // Packed leaf coordinates (see ct_pack_leaf_coords).
 __global__ void __launch_bounds__(kBlock) k_pack_leaf(TreeView t, double* __restrict__ out) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -450,7 +458,7 @@ __global__ void __launch_bounds__(kBlock) k_boxes(TreeView t, const double* __re
       if (exhausted) break;
       for (int k = nd.ptr; k < nd.ptr + nd.size; ++k) {
         const int c = t.bb_indices[k];
-        if (boxes_intersect(box, ld_box(t.bb, c))) {
+        if (boxes_intersect(box, leaf_box(t, k, c))) {
           if (WRITE) {
             ii[o] = q;
             jj[o] = c;
@@ -926,6 +934,16 @@ int ct_pack_leaf_coords(const ct_tree* tree, double* out, void* stream) {
   if (rc) return rc;
   CT_REQUIRE(tree->n_max_vert >= 3 && tree->n_max_vert <= 8 && out, "n_max_vert must be in [3, 8]");
   k_pack_leaf<<<grid_for(tree->n_elements), kBlock, 0, as_stream(stream)>>>(view(tree), out);
+  CT_LAUNCH_CHECK();
+  return CT_OK;
+}
+
+int ct_pack_leaf_bboxes(const ct_tree* tree, double* out, void* stream) {
+  int rc = check_tree(tree);
+  if (rc) return rc;
+  CT_REQUIRE(out != nullptr, "null output");
+  if (tree->n_elements == 0) return CT_OK;
+  k_pack_leaf_bb<<<grid_for(tree->n_elements), kBlock, 0, as_stream(stream)>>>(view(tree), out);
   CT_LAUNCH_CHECK();
   return CT_OK;
 }

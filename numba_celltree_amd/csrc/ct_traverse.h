@@ -29,13 +29,14 @@ struct TreeView {
   const int32_t* grid;  // start-node grid (grid_n x grid_n over bbox), or null
   int grid_n;
   double gsx, gsy;      // grid_n / bbox extent, per axis
+  const double* lbb;    // cell bboxes in bb_indices order, or null
 };
 
 __host__ __device__ __forceinline__ TreeView view(const ct_tree& t) {
   const int g = t.start_grid ? t.start_grid_n : 0;
   return TreeView{t.vertices, t.elements, t.bb_coords, t.nodes, t.bb_indices, t.leaf_coords, t.node_boxes,
                   t.n_max_vert, (int)t.n_elements, Box{t.bbox[0], t.bbox[1], t.bbox[2], t.bbox[3]},
-                  t.start_grid, g, g / (t.bbox[1] - t.bbox[0]), g / (t.bbox[3] - t.bbox[2])};
+                  t.start_grid, g, g / (t.bbox[1] - t.bbox[0]), g / (t.bbox[3] - t.bbox[2]), t.leaf_bb};
 }
 
 inline TreeView view(const ct_tree* t) { return view(*t); }
@@ -266,14 +267,26 @@ __device__ __forceinline__ int locate_one(const TreeView& t, P2 p, double tol, L
 // ----------------------------------------------------------------- segments
 // query.py:309-328 compute_edge_face_intersect: Cohen-Sutherland on the cell bbox, then
 // Cyrus-Beck on the cell with tol = max(1e-15, 1e-12 * max(w, h)).
+// `box` is cell c's bbox.
 template <int MAXV>
-__device__ __forceinline__ bool edge_face(const TreeView& t, int c, P2 a, P2 b, P2* cc, P2* dd) {
-  const Box box = ld_box(t.bb, c);
+__device__ __forceinline__ bool edge_face_box(const TreeView& t, int c, const Box& box, P2 a, P2 b,
+                                              P2* cc, P2* dd) {
   if (!cohen_sutherland(a, b, box, cc, dd)) return false;
   P2 poly[MAXV];
   const int len = load_poly<MAXV>(t.elements, t.nmax, c, t.vertices, poly);
   const double tol = pmax(kMinTolerance, kToleranceFactor * pmax(box.xmax - box.xmin, box.ymax - box.ymin));
   return cyrus_beck<MAXV>(a, b, poly, len, tol, cc, dd);
+}
+
+template <int MAXV>
+__device__ __forceinline__ bool edge_face(const TreeView& t, int c, P2 a, P2 b, P2* cc, P2* dd) {
+  return edge_face_box<MAXV>(t, c, ld_box(t.bb, c), a, b, cc, dd);
+}
+
+// The bbox of the cell at leaf position k (cell c = bb_indices[k]): the leaf-ordered copy
+// when the tree has one (a read that does not wait for c).
+__device__ __forceinline__ Box leaf_box(const TreeView& t, int k, int c) {
+  return t.lbb ? ld_box(t.lbb, k) : ld_box(t.bb, c);
 }
 
 // query.py:292-306 compute_edge_edge_intersect: (c, c).
@@ -403,7 +416,9 @@ __device__ __forceinline__ int64_t edge_query(const TreeView& t, P2 a, P2 b, int
     for (int k = nd.ptr; k < nd.ptr + nd.size; ++k) {
       const int c = t.bb_indices[k];
       P2 cc, dd;
-      const bool hit = FACES ? edge_face<MAXV>(t, c, a, b, &cc, &dd) : edge_edge(t, c, a, b, &cc, &dd);
+      bool hit;
+      if constexpr (FACES) hit = edge_face_box<MAXV>(t, c, leaf_box(t, k, c), a, b, &cc, &dd);
+      else hit = edge_edge(t, c, a, b, &cc, &dd);
       if (hit) {
         if (WRITE) {
           ii[o + cnt] = q;

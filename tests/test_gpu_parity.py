@@ -446,6 +446,36 @@ def test_intersect_edges_ordered(trees):
     assert_close(gxy, oxy, L)
 
 
+def test_leaf_bboxes_do_not_change_results(trees):
+    """Box and segment queries with and without the leaf-ordered cell boxes: identical rows
+    (bit for bit), and the oracle's."""
+    name, v, f, g, o = trees
+    L = _extent(v)
+    h = L / np.sqrt(len(f))
+    bx = S.boxes(5_000, L, h, 0.1, 10.0, 8) + np.array([v[:, 0].min()] * 2 + [v[:, 1].min()] * 2)
+    seg = S.segments(5_000, L, h, 0.1, 20.0, 9) + v.min(0)[None, None, :]
+    assert g._ctree.leaf_bb
+    saved = g._ctree.leaf_bb
+    got = {}
+    try:
+        for on in (True, False):
+            g._ctree.leaf_bb = saved if on else None
+            log(f"leaf bboxes {name} on={on}")
+            got[on] = (g.locate_boxes(bx), g.intersect_edges(seg))
+    finally:
+        g._ctree.leaf_bb = saved
+    (bi, bj), (ei, ej, exy) = got[True]
+    (ci, cj), (fi, fj, fxy) = got[False]
+    assert np.array_equal(bi, ci) and np.array_equal(bj, cj)
+    assert np.array_equal(ei, fi) and np.array_equal(ej, fj)
+    assert np.array_equal(exy.view(np.int64), fxy.view(np.int64))
+    oi, oj = o.locate_boxes(bx)
+    assert np.array_equal(bi, oi) and np.array_equal(bj, oj)
+    oi, oj, oxy = o.intersect_edges(seg)
+    assert np.array_equal(ei, oi) and np.array_equal(ej, oj)
+    assert_close(exy, oxy, L)
+
+
 def test_intersect_faces(trees):
     name, v, f, g, o = trees
     L = _extent(v)
