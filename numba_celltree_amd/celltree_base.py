@@ -26,7 +26,7 @@ START_GRID_N = None
 # Every tree also keeps its cell boxes in bb_indices order (ct_pack_leaf_bboxes, 32 bytes per
 # cell) for box and segment queries; False builds trees without them (same results).
 LEAF_BBOXES = True
-START_GRID_MAX = 2048
+START_GRID_MAX = 1024
 
 
 def start_grid_size(n_nodes: int) -> int:
