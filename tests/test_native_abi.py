@@ -90,3 +90,45 @@ def test_no_gpu_fails_loudly():
 
     with pytest.raises(RuntimeError, match="ROCm GPU"):
         M.CellTree2d([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]], [[0, 1, 2]], -1)
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Every ctypes mirror of a header struct has the C compiler's size and field offsets."""
+    import shutil
+    import subprocess
+
+    from numba_celltree_amd import _native
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    mirrors = {"ct_node": _native.CtNode, "ct_tree": _native.CtTree, "ct_batch_tile": _native.CtBatchTile,
+               "ct_batch": _native.CtBatch, "ct_regrid_topology": _native.CtRegridTopology}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
+    for cname, py in mirrors.items():
+        lines.append(f'  printf("{cname} @sizeof %zu\\n", sizeof({cname}));')
+        for field, _ in py._fields_:
+            lines.append(f'  printf("{cname} {field} %zu\\n", offsetof({cname}, {field}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, py in mirrors.items():
+        assert got[(cname, "@sizeof")] == ctypes.sizeof(py), cname
+        for field, _ in py._fields_:
+            assert got[(cname, field)] == getattr(py, field).offset, (cname, field)
+
+
+def test_start_grid_size_rule():
+    from numba_celltree_amd import celltree_base as CB
+
+    assert CB.START_GRID_N is None
+    assert CB.start_grid_size(10) == 0 and CB.start_grid_size(63) == 0
+    assert CB.start_grid_size(64) == 8
+    assert CB.start_grid_size(1000) == 32  # 31.6 -> 32
+    assert CB.start_grid_size(1024) == 32
+    assert CB.start_grid_size(1025) == 64
+    assert CB.start_grid_size(1_201_919) == CB.START_GRID_MAX == 1024
