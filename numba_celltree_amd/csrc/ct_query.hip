@@ -74,7 +74,10 @@ struct OrderGeom {
 __device__ __forceinline__ unsigned order_key(int kind, const double* __restrict__ q, int64_t i,
                                               const OrderGeom& g) {
   if (kind == 0) {
-    const double2 p = reinterpret_cast<const double2*>(q)[i];
+    // streaming: the sort reads each point once, and a cached copy would push the tree out
+    // of the Infinity Cache before the traversal
+    const double px = __builtin_nontemporal_load(q + 2 * i), py = __builtin_nontemporal_load(q + 2 * i + 1);
+    const double2 p = make_double2(px, py);
     return spread_bits(grid_coord(p.x, g.bb.xmin, g.sx, 1024.0)) |
            (spread_bits(grid_coord(p.y, g.bb.ymin, g.sy, 1024.0)) << 1);
   }
@@ -288,7 +291,8 @@ __global__ void __launch_bounds__(kChunkBlock) k_sort_chunks(int kind, int64_t n
       copy_row(kind, cq, src, sorted, lo + pos);
     }
   } else {
-    for (int pos = threadIdx.x; pos < m; pos += kChunkBlock) order[lo + pos] = staged[pos];
+    // streaming store: the traversal reads the order once, with a streaming load
+    for (int pos = threadIdx.x; pos < m; pos += kChunkBlock) __builtin_nontemporal_store(staged[pos], order + lo + pos);
   }
 }
 
