@@ -1,0 +1,83 @@
+"""The plain GPU workload at its full size against the oracle: the 999,941-cell mixed
+triangle/quad mesh (708 x 708 jittered grid on [0, 1000]^2), 1e7 points (5% outside the
+bbox), 1e6 boxes and 1e6 segments, exactly as the benchmark generates them.
+
+Bar: the tree byte-identical; every point's cell, every (box, cell) pair and every (segment,
+cell) row identical, in the reference's order; intersection points within 1e-12 relative of
+the domain size.  The oracle takes about 15-30 s of host time for all of it.
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+from numba_celltree_amd import synthetic as S
+from oracle import refcpu as R
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(f"[gpu fullsize] {msg}", file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module")
+def full():
+    import torch
+
+    import numba_celltree_amd as M
+
+    log("generating the mesh and queries")
+    v, f = S.config2_mesh()
+    pts, bx, sg = S.config2_queries(10_000_000, 1_000_000, 1_000_000)
+    log("building the trees")
+    g = M.CellTree2d(torch.from_numpy(v).cuda(), torch.from_numpy(f).cuda(), -1)
+    o = R.OracleCellTree2d(v, f, -1)
+    return v, f, pts, bx, sg, g, o
+
+
+def test_full_tree_identical(full):
+    v, f, pts, bx, sg, g, o = full
+    gn, on = g.nodes, o.nodes
+    assert gn.shape == on.shape
+    for field in ("child", "ptr", "size", "dim"):
+        assert np.array_equal(gn[field], on[field]), field
+    for field in ("Lmax", "Rmin"):
+        assert np.array_equal(gn[field].view(np.int64), on[field].view(np.int64)), field
+    assert np.array_equal(g.bb_indices, o.bb_indices)
+
+
+def test_full_locate_points(full):
+    import torch
+
+    v, f, pts, bx, sg, g, o = full
+    log("locate_points 1e7")
+    got = g.locate_points(torch.from_numpy(pts).cuda()).cpu().numpy()
+    want = o.locate_points(pts)
+    assert np.array_equal(got, want)
+    assert 0.9 < (want >= 0).mean() < 1.0  # the outside points are there
+
+
+def test_full_locate_boxes(full):
+    import torch
+
+    v, f, pts, bx, sg, g, o = full
+    log("locate_boxes 1e6")
+    gi, gj = (x.cpu().numpy() for x in g.locate_boxes(torch.from_numpy(bx).cuda()))
+    oi, oj = o.locate_boxes(bx)
+    assert len(gi) == len(oi) > 10_000_000
+    assert np.array_equal(gi, oi) and np.array_equal(gj, oj)
+
+
+def test_full_intersect_edges(full):
+    import torch
+
+    v, f, pts, bx, sg, g, o = full
+    log("intersect_edges 1e6")
+    gi, gj, gxy = (x.cpu().numpy() for x in g.intersect_edges(torch.from_numpy(sg).cuda()))
+    oi, oj, oxy = o.intersect_edges(sg)
+    assert len(gi) == len(oi) > 1_000_000
+    assert np.array_equal(gi, oi) and np.array_equal(gj, oj)
+    err = np.abs(gxy - oxy)
+    assert np.all(err <= 1e-12 * np.maximum(np.abs(oxy), 1000.0))
