@@ -1,6 +1,7 @@
-"""The plain GPU workload at its full size against the oracle: the 999,941-cell mixed
-triangle/quad mesh (708 x 708 jittered grid on [0, 1000]^2), 1e7 points (5% outside the
-bbox), 1e6 boxes and 1e6 segments, exactly as the benchmark generates them.
+"""The plain GPU workloads at their full size against the oracle: the 999,941-cell mixed
+triangle/quad mesh (708 x 708 jittered grid on [0, 1000]^2) with 1e7 points (5% outside the
+bbox), 1e6 boxes and 1e6 segments, and intersect_faces of 999,698 triangles against 250,000
+Voronoi cells, exactly as the benchmark generates them.
 
 Bar: the tree byte-identical; every point's cell, every (box, cell) pair and every (segment,
 cell) row identical, in the reference's order; intersection points within 1e-12 relative of
@@ -81,3 +82,25 @@ def test_full_intersect_edges(full):
     assert np.array_equal(gi, oi) and np.array_equal(gj, oj)
     err = np.abs(gxy - oxy)
     assert np.all(err <= 1e-12 * np.maximum(np.abs(oxy), 1000.0))
+
+
+def test_full_intersect_faces():
+    """intersect_faces at full size: 999,698 jittered-grid triangles against 250,000 hex
+    Voronoi cells (3 to 7 vertices): the tree identical, every (target, source) pair
+    identical in the reference's order, overlap areas within 1e-12 relative."""
+    import torch
+
+    import numba_celltree_amd as M
+
+    log("intersect_faces 999,698 x 250,000")
+    sv, sf, tv, tf, _ = S.config3()
+    g = M.CellTree2d(torch.from_numpy(sv).cuda(), torch.from_numpy(sf).cuda(), -1)
+    o = R.OracleCellTree2d(sv, sf, -1)
+    assert np.array_equal(g.bb_indices, o.bb_indices)
+    gi, gj, ga = (x.cpu().numpy() for x in g.intersect_faces(torch.from_numpy(tv).cuda(),
+                                                             torch.from_numpy(tf).cuda(), -1))
+    oi, oj, oa = o.intersect_faces(tv, tf, -1)
+    assert len(gi) == len(oi) > 2_000_000
+    assert np.array_equal(gi, oi) and np.array_equal(gj, oj)
+    h2 = (1000.0 / 707) ** 2  # a source cell's area scale
+    assert np.all(np.abs(ga - oa) <= 1e-12 * np.maximum(np.abs(oa), h2))
