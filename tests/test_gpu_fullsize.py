@@ -129,3 +129,18 @@ def test_full_regrid_forward(full):
     np.add.at(y_ref, i, area[:, None] * F[j])
     y_ref /= tarea[:, None]
     assert np.allclose(y, y_ref, rtol=1e-12, atol=1e-12 * np.abs(F).max())
+
+
+def test_full_barycentric_weights():
+    """compute_barycentric_weights for 1e6 points on the 20,000-triangle mesh (uniform over
+    [-0.5, 10.5]^2, so some fall outside): cells identical, weights within 1e-12."""
+    import numba_celltree_amd as M
+
+    log("barycentric 1e6 points")
+    v, f, _ = S.config1()
+    pts = np.random.default_rng(4).uniform(-0.5, 10.5, size=(1_000_000, 2))
+    gf, gw = M.CellTree2d(v, f, -1).compute_barycentric_weights(pts)
+    of, ow = R.OracleCellTree2d(v, f, -1).compute_barycentric_weights(pts)
+    assert np.array_equal(gf, of)
+    assert (of == -1).any() and (of >= 0).mean() > 0.8
+    assert np.all(np.abs(gw - ow) <= 1e-12 * np.maximum(np.abs(ow), 1.0))
