@@ -189,7 +189,7 @@ int ct_locate_points_on_edge(const ct_tree* tree, const double* points, int64_t 
  * points by the Morton key of a 1024 x 1024 grid over `bbox` (HOST array of 4); boxes and
  * segments by a work class (log2 of their bbox area relative to `bbox`) above the Morton key
  * of their centre.  The order is grouped by the top 8 key bits and sorted by the full key
- * within chunks of 16384 queries of one group (a group larger than that is a sequence of
+ * within chunks of 32768 queries of one group (a group larger than that is a sequence of
  * such sorted chunks).  Passing the order to a count / write pass gives each wavefront
  * neighbouring queries of similar cost; results are unchanged by it.  `sorted` (may be NULL)
  * receives the queries in that order (n rows of 2 or 4 doubles). */

@@ -120,8 +120,10 @@ __device__ __forceinline__ unsigned order_key(int kind, const double* __restrict
 // Every stage reads and writes each entry once.  A coarse bin (1/256 of the bbox) larger
 // than kChunk is ordered chunk by chunk, so the order is exact within chunks of kChunk
 // queries and interleaves them within the bin.  Order only schedules lanes: the traversal
-// needs a wavefront's queries to be close, and 64 consecutive entries of a 16384-query
-// chunk of one coarse cell still span about 1/256 of that cell.
+// needs a wavefront's queries to be close, and 64 consecutive entries of a 32768-query
+// chunk of one coarse cell still span about 1/512 of that cell.  Chunks of 32768 (128 KB of
+// staged indices, one workgroup per CU) ordered 1e7 points 6 us slower than chunks of 16384
+// but the traversal ran 12 us faster; chunks of 8192 were slower in both.
 constexpr int kSortItems = 32;
 constexpr int kSortBatch = 8;  // loads in flight per lane
 constexpr int kSortTile = kBlock * kSortItems;
@@ -129,7 +131,7 @@ constexpr int kCoarse = 256;  // == kBlock
 constexpr int kFineBits = 12;
 constexpr int kFine = 1 << kFineBits;
 constexpr int kChunkBlock = 1024;
-constexpr int kChunkItems = 16;  // entries per lane, held in registers
+constexpr int kChunkItems = 32;  // entries per lane, held in registers
 constexpr int kChunk = kChunkBlock * kChunkItems;
 
 __global__ void __launch_bounds__(kBlock) k_sort_hist(int kind, const double* __restrict__ q, int64_t n,

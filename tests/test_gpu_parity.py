@@ -384,7 +384,7 @@ def _point_keys(q, bbox):
 
 def test_query_order_groups_and_chunks():
     """The point order is grouped by the top 8 key bits and sorted by the full key within
-    every chunk of 16384 entries of a group; a clustered set makes groups of many chunks."""
+    every chunk of 32768 entries of a group; a clustered set makes groups of many chunks."""
     import torch
 
     from numba_celltree_amd import _device as D
@@ -393,7 +393,7 @@ def test_query_order_groups_and_chunks():
     bbox = [0.0, 100.0, 0.0, 100.0]
     q = np.concatenate([
         rng.uniform(-5, 105, size=(150_000, 2)),
-        rng.normal(30.0, 0.5, size=(70_000, 2)),  # ~4 chunks in one or two coarse groups
+        rng.normal(30.0, 0.5, size=(140_000, 2)),  # ~4 chunks in one or two coarse groups
         rng.uniform(0, 100, size=(1, 2)),
     ])
     q[17] = np.nan
@@ -407,8 +407,8 @@ def test_query_order_groups_and_chunks():
     n_chunks = 0
     for c in np.unique(coarse):
         lo, hi = np.searchsorted(coarse, c, "left"), np.searchsorted(coarse, c, "right")
-        for s in range(lo, hi, 16384):
-            seg = k[s:min(s + 16384, hi)]
+        for s in range(lo, hi, 32768):
+            seg = k[s:min(s + 32768, hi)]
             assert np.all(np.diff(seg.astype(np.int64)) >= 0), (c, s)
             n_chunks += 1
     assert n_chunks > len(np.unique(coarse))  # some group holds several chunks
