@@ -63,15 +63,34 @@ def to_dev(x, dtype: torch.dtype) -> torch.Tensor:
     return t
 
 
-def faces_to_dev(faces) -> torch.Tensor:
-    """int32 device copy of a (n, n_max_vert) index array."""
+def faces_to_dev(faces, n_vertices: int, allow_fill: bool = True) -> torch.Tensor:
+    """int32 device copy of a (n, n_max_vert) index array.
+
+    Every index must address one of ``n_vertices`` vertices; -1 (the fill value) is allowed
+    only where ``allow_fill`` (face rows; edge rows have no fill).  The kernels read vertices
+    without bounds checks, so this is checked here, before any launch."""
     if isinstance(faces, torch.Tensor):
         f = faces
     else:
         f = torch.from_numpy(np.ascontiguousarray(faces))
-    if f.numel() and (int(f.max()) >= 2**31 or int(f.min()) < -1):
-        raise ValueError("element indices must lie in [-1, 2**31)")
+    if f.numel():
+        hi, lo = int(f.max()), int(f.min())
+        if hi >= min(int(n_vertices), 2**31):
+            raise ValueError(f"element index {hi} is out of bounds for {int(n_vertices)} vertices")
+        if lo < (-1 if allow_fill else 0):
+            what = "[-1, n_vertices)" if allow_fill else "[0, n_vertices)"
+            raise ValueError(f"element indices must lie in {what}, found {lo}")
     return f.to(device=device(), dtype=torch.int32).contiguous()
+
+
+def index_to_dev(x, n: int, allow_missing: bool = False) -> torch.Tensor:
+    """int64 device copy of an index array into n items (-1 allowed where ``allow_missing``)."""
+    t = x if isinstance(x, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(x))
+    if t.numel():
+        hi, lo = int(t.max()), int(t.min())
+        if hi >= int(n) or lo < (-1 if allow_missing else 0):
+            raise ValueError(f"indices must lie in [{-1 if allow_missing else 0}, {int(n)}), found [{lo}, {hi}]")
+    return to_dev(t, torch.int64)
 
 
 def empty(shape, dtype) -> torch.Tensor:

@@ -12,11 +12,22 @@ import torch
 
 from numba_celltree_amd import _device as D
 from numba_celltree_amd import _native
+from numba_celltree_amd.regrid import _as_f64, _require
 
 
 class BarycentricTriangleFunction(torch.autograd.Function):
+    """Triangle weights of points (n, 2) in faces face_index (n,) of a mesh (vertices
+    (n_vertices, 2), faces (n_faces, 3) int32), all on the current GPU; points and vertices
+    float64.  ``barycentric_weights`` converts and checks its inputs."""
+
     @staticmethod
     def forward(ctx, points, vertices, faces, face_index, tolerance: float):
+        _require("points", points, (None, 2))
+        _require("vertices", vertices, (None, 2))
+        if faces.dtype != torch.int32 or faces.ndim != 2 or int(faces.shape[1]) != 3 or not faces.is_cuda:
+            raise ValueError("faces must be a (n_faces, 3) int32 device tensor")
+        if face_index.dtype != torch.int64 or face_index.shape != points.shape[:1] or not face_index.is_cuda:
+            raise ValueError("face_index must be an int64 device tensor with one entry per point")
         points = points.contiguous()
         vertices = vertices.contiguous()
         n = int(points.shape[0])
@@ -33,7 +44,7 @@ class BarycentricTriangleFunction(torch.autograd.Function):
         gp = D.empty((n, 2), torch.float64) if ctx.needs_input_grad[0] else None
         gv = torch.zeros_like(vertices) if ctx.needs_input_grad[1] else None
         _native.call("ct_barycentric_triangle_backward", D.ptr(points), D.ptr(face_index), n, D.ptr(faces),
-                     int(faces.shape[1]), D.ptr(vertices), D.ptr(gw.contiguous()), D.ptr(gp), D.ptr(gv),
+                     int(faces.shape[1]), D.ptr(vertices), D.ptr(gw.to(torch.float64).contiguous()), D.ptr(gp), D.ptr(gv),
                      D.stream())
         return gp, gv, None, None, None
 
@@ -48,6 +59,10 @@ def barycentric_weights(tree, points: torch.Tensor, vertices: Optional[torch.Ten
         tolerance = tree._default_tolerance()
     if vertices is None:
         vertices = tree.d_vertices
+    # Host arrays, other float dtypes and other devices are converted (differentiably).
+    points, vertices = (_as_f64(x) for x in (points, vertices))
+    _require("points", points, (None, 2))
+    _require("vertices", vertices, (int(tree.d_vertices.shape[0]), 2))
     face = tree._locate_points_dev(points.detach().contiguous(), float(tolerance))
     w = BarycentricTriangleFunction.apply(points, vertices, tree.d_elements, face, float(tolerance))
     return face, w

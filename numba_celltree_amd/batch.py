@@ -30,6 +30,7 @@ import torch
 
 from numba_celltree_amd import _device as D
 from numba_celltree_amd import _native
+from numba_celltree_amd.celltree_base import CellTree2dBase
 
 STACK_CLASSES = (16, 24, 32, 48, 64, 128, 256)
 TILE_DTYPE = np.dtype([("begin", "<i8"), ("count", "<i4"), ("tree", "<i4")])
@@ -249,11 +250,10 @@ class EdgeBuffers:
     count pass's hit record (None below 4 slots per segment)."""
 
     def __init__(self, n_queries: int, capacity: int, record: bool = True):
-        slots = min(EDGE_HIT_SLOTS, EDGE_HIT_BUDGET // max(n_queries, 1)) if record else 0
-        self.slots = slots if slots >= 4 else 0
-        self.hits = D.empty(self.slots * n_queries, torch.int32) if self.slots and n_queries else None
-        if self.hits is None:
-            self.slots = 0
+        self.hits, self.slots = None, 0
+        if record and n_queries:
+            # Fewer slots (down to none) when the device cannot hold the full record.
+            self.hits, self.slots = CellTree2dBase._hit_record(n_queries, EDGE_HIT_SLOTS, EDGE_HIT_BUDGET)
         self.capacity = int(capacity)
         self.counts = D.empty(n_queries, torch.int64)
         self.offsets = D.empty(n_queries + 1, torch.int64)

@@ -39,7 +39,7 @@ class CellTree2d(CellTree2dBase):
         vertices = cast_vertices(vertices, copy=True)
         faces = cast_faces(faces, fill_value)
         d_vertices = D.to_dev(vertices, torch.float64)
-        d_faces = D.faces_to_dev(faces)
+        d_faces = D.faces_to_dev(faces, d_vertices.shape[0])
         n, nmax = int(d_faces.shape[0]), int(d_faces.shape[1])
         if n == 0:
             raise ValueError("faces is empty")
@@ -91,8 +91,7 @@ class CellTree2d(CellTree2dBase):
         st = D.stream()
         order, srt = self._order(1, d_boxes)
         counts = D.empty(n, torch.int64)
-        slots = self._box_hit_slots(n)
-        hits = D.empty(slots * max(n, 1), torch.int32) if slots > 0 else None
+        hits, slots = self._hit_record(n, self.BOX_HIT_SLOTS, self.BOX_HIT_BUDGET)
         _native.call("ct_locate_boxes_count_rec", self.tree_ref, D.ptr(d_boxes), n, D.ptr(order), D.ptr(srt),
                      D.ptr(counts), D.ptr(hits), slots, st)
         offsets, total = D.exclusive_scan(counts)
@@ -166,7 +165,7 @@ class CellTree2d(CellTree2dBase):
         """
         io = D.Io(vertices, faces)
         d_vertices = D.to_dev(cast_vertices(vertices), torch.float64)
-        d_faces = D.faces_to_dev(faces)
+        d_faces = D.faces_to_dev(faces, d_vertices.shape[0])
         i, j, _ = self._face_pairs(d_vertices, d_faces, with_area=False)
         if isinstance(faces, np.ndarray) and faces.ndim == 2:
             faces[...] = d_faces.cpu().numpy().astype(faces.dtype)
@@ -178,7 +177,7 @@ class CellTree2d(CellTree2dBase):
         """Face pairs with overlap area > 0, and the area (celltree.py:228-269)."""
         io = D.Io(vertices, faces)
         d_vertices = D.to_dev(cast_vertices(vertices), torch.float64)
-        d_faces = D.faces_to_dev(cast_faces(faces, fill_value))
+        d_faces = D.faces_to_dev(cast_faces(faces, fill_value), d_vertices.shape[0])
         i, j, a = self._face_pairs(d_vertices, d_faces, with_area=True)
         return io.out(i), io.out(j), io.out(a)
 

@@ -250,17 +250,25 @@ class CellTree2dBase:
 
     # Box queries: a box's hits are its output rows, and wide boxes have hundreds (cfg2: up to
     # ~900), so the record is larger: up to BOX_HIT_SLOTS slots within BOX_HIT_BUDGET int32
-    # entries (4 GiB).
+    # entries (4 GiB).  All four are class defaults; setting them on a tree instance changes
+    # that tree only (results never depend on them).
     BOX_HIT_SLOTS = 1024
     BOX_HIT_BUDGET = 1 << 30
 
-    def _edge_hit_slots(self, n: int) -> int:
-        slots = min(int(self.EDGE_HIT_SLOTS), int(self.EDGE_HIT_BUDGET) // max(n, 1))
-        return slots if slots >= 4 else 0
-
-    def _box_hit_slots(self, n: int) -> int:
-        slots = min(int(self.BOX_HIT_SLOTS), int(self.BOX_HIT_BUDGET) // max(n, 1))
-        return slots if slots >= 4 else 0
+    @staticmethod
+    def _hit_record(n: int, max_slots: int, budget: int):
+        """(hits, slots): a record of up to max_slots slots per query within budget int32
+        entries.  When the device cannot hold it, fewer slots are tried, down to no record
+        (slots 0: the write pass walks the tree again), so memory pressure costs speed, not
+        the call."""
+        slots = min(int(max_slots), int(budget) // max(n, 1))
+        while slots >= 4:
+            try:
+                return D.empty(slots * max(n, 1), torch.int32), slots
+            except torch.cuda.OutOfMemoryError:
+                torch.cuda.empty_cache()
+                slots //= 2
+        return None, 0
 
     def _locate_edges(self, edge_coords, kind: int):
         """count (recording hits) -> scan -> write, then the per-segment sort by t (device
@@ -269,8 +277,7 @@ class CellTree2dBase:
         st = D.stream()
         order, srt = self._order(2, edge_coords)
         counts = D.empty(n, torch.int64)
-        slots = self._edge_hit_slots(n)
-        hits = D.empty(slots * max(n, 1), torch.int32) if slots > 0 else None  # 0: no record
+        hits, slots = self._hit_record(n, self.EDGE_HIT_SLOTS, self.EDGE_HIT_BUDGET)
         _native.call(
             "ct_locate_edges_count_rec", self.tree_ref, D.ptr(edge_coords), n, kind, D.ptr(order), D.ptr(srt),
             D.ptr(counts), D.ptr(hits), slots, st

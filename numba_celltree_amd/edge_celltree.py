@@ -26,7 +26,7 @@ class EdgeCellTree2d(CellTree2dBase):
             raise ValueError("cells_per_leaf must be >= 1")
         vertices = cast_vertices(vertices, copy=True)
         d_vertices = D.to_dev(vertices, torch.float64)
-        d_edges = D.faces_to_dev(edges)
+        d_edges = D.faces_to_dev(edges, d_vertices.shape[0], allow_fill=False)
         if d_edges.ndim != 2 or d_edges.shape[1] != 2:
             raise ValueError("edges must have shape (n_edge, 2)")
         st = D.stream()

@@ -46,7 +46,7 @@ class RegridTopology:
     def __init__(self, tree, tgt_vertices, tgt_faces, fill_value: int = -1):
         self.tree = tree
         d_tv = D.to_dev(cast_vertices(tgt_vertices), torch.float64)
-        d_tf = D.faces_to_dev(cast_faces(tgt_faces, fill_value))
+        d_tf = D.faces_to_dev(cast_faces(tgt_faces, fill_value), d_tv.shape[0])
         if int(d_tf.shape[1]) > 8 or int(tree.d_elements.shape[1]) > 8:
             raise ValueError("regridding supports faces of at most 8 vertices")
         # intersect_faces reorders the target rows counter-clockwise in place (d_tf) and
@@ -122,11 +122,30 @@ class RegridTopology:
         return g_sv, g_tv
 
 
+def _require(name: str, t, shape) -> None:
+    """Raise unless ``t`` is a float64 tensor on the current GPU with ``shape`` (None: any)."""
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch.Tensor")
+    if t.dtype != torch.float64:
+        raise TypeError(f"{name} must be float64, got {t.dtype}")
+    if not t.is_cuda or t.device != D.device():
+        raise ValueError(f"{name} must be on {D.device()}, got {t.device}")
+    if t.ndim != len(shape) or any(want is not None and int(got) != want for got, want in zip(t.shape, shape)):
+        raise ValueError(f"{name} must have shape {tuple(shape)}, got {tuple(t.shape)}")
+
+
 class RegridFunction(torch.autograd.Function):
-    """y = regrid(F, src_vertices, tgt_vertices) under a fixed RegridTopology."""
+    """y = regrid(F, src_vertices, tgt_vertices) under a fixed RegridTopology.
+
+    F (n_src, n_fields), src_vertices (n_src_vertices, 2) and tgt_vertices
+    (n_tgt_vertices, 2) must be float64 tensors on the current GPU (``Regridder`` converts
+    other inputs)."""
 
     @staticmethod
     def forward(ctx, F, src_vertices, tgt_vertices, topo: RegridTopology):
+        _require("F", F, (topo.n_src, None))
+        _require("src_vertices", src_vertices, (topo.n_src_vertices, 2))
+        _require("tgt_vertices", tgt_vertices, (topo.n_tgt_vertices, 2))
         F = F.contiguous()
         src_vertices = src_vertices.contiguous()
         tgt_vertices = tgt_vertices.contiguous()
@@ -141,12 +160,18 @@ class RegridFunction(torch.autograd.Function):
     def backward(ctx, gy):
         F, src_v, tgt_v, area, tarea, y = ctx.saved_tensors
         topo = ctx.topo
-        gy = gy.contiguous()
+        gy = gy.to(dtype=torch.float64).contiguous()
         gF = topo.apply_transpose(area, tarea, gy) if ctx.needs_input_grad[0] else None
         g_sv = g_tv = None
         if ctx.needs_input_grad[1] or ctx.needs_input_grad[2]:
             g_sv, g_tv = topo.vertex_backward(src_v, tgt_v, F, y, gy, tarea)
         return gF, g_sv, g_tv, None
+
+
+def _as_f64(x) -> torch.Tensor:
+    if not isinstance(x, torch.Tensor):
+        x = torch.as_tensor(x)
+    return x.to(device=D.device(), dtype=torch.float64)
 
 
 class Regridder:
@@ -167,6 +192,9 @@ class Regridder:
             src_vertices = topo.tree.d_vertices
         if tgt_vertices is None:
             tgt_vertices = topo.tgt_vertices
+        # Host arrays, other float dtypes and other devices are converted (differentiably);
+        # shapes are checked by RegridFunction before any kernel runs.
+        F, src_vertices, tgt_vertices = (_as_f64(x) for x in (F, src_vertices, tgt_vertices))
         squeeze = F.ndim == 1
         if squeeze:
             F = F[:, None]

@@ -483,3 +483,73 @@ MODULE_CASES = [
     check_barycentric_module_weights, check_counter_clockwise, check_build_bboxes,
     check_sort_intersections_by_edge,
 ]
+
+
+# -------------------------------------------------------------- test_celltree_utils.py
+UTIL_BBOX_COORDS = np.array(  # test_celltree_utils.py:6-16
+    [
+        [1.0, 2.0, 1.0, 2.0], [4.0, 5.0, 0.0, 1.0], [4.0, 5.0, 2.0, 3.0], [-1.0, 0.0, 0.0, 4.0],
+        [6.0, 8.0, 0.0, 4.0], [0.0, 6.0, -1.0, 0.0], [0.0, 6.0, 4.0, 5.0],
+    ]
+)
+MIN_TOLERANCE = 1e-15  # constants.py
+TOLERANCE_FACTOR = 1e-12
+
+
+def check_default_tolerance(impl):  # test_celltree_utils.py:19-30
+    bb_diagonal = np.array([2.4, 0.5])
+    np.testing.assert_allclose(
+        impl.default_tolerance(bb_diagonal), 2.4 * TOLERANCE_FACTOR, rtol=0, atol=MIN_TOLERANCE / 1e5
+    )
+    np.testing.assert_allclose(
+        impl.default_tolerance(bb_diagonal / 1e4), MIN_TOLERANCE, rtol=0, atol=MIN_TOLERANCE / 1e5
+    )
+
+
+def check_bbox_tree(impl):  # test_celltree_utils.py:33-36
+    np.testing.assert_allclose(impl.bbox_tree(UTIL_BBOX_COORDS), [-1.0, 8.0, -1.0, 5.0], rtol=0, atol=1e-5)
+
+
+def check_bbox_distances(impl):  # test_celltree_utils.py:39-52
+    expected = np.array(
+        [
+            [1.0, 1.0, 1.41421356], [1.0, 1.0, 1.41421356], [1.0, 1.0, 1.41421356], [1.0, 4.0, 4.12310563],
+            [2.0, 4.0, 4.47213595], [6.0, 1.0, 6.08276253], [6.0, 1.0, 6.08276253],
+        ]
+    )
+    np.testing.assert_allclose(impl.bbox_distances(UTIL_BBOX_COORDS), expected, rtol=0, atol=1e-5)
+
+
+def check_bbox_distances_pythagoras(impl):
+    """Mathematics, not a reference vector: the third column is the Euclidean diagonal
+    (correctly rounded sqrt of dx^2 + dy^2 agrees with hypot to 1 ulp), on boxes spanning
+    many magnitudes, including zero-width ones."""
+    rng = np.random.default_rng(11)
+    lo = rng.uniform(-1e3, 1e3, (500, 2))
+    ext = 10.0 ** rng.uniform(-6, 4, (500, 2))
+    ext[:5, 0] = 0.0
+    bb = np.column_stack([lo[:, 0], lo[:, 0] + ext[:, 0], lo[:, 1], lo[:, 1] + ext[:, 1]])
+    d = np.asarray(impl.bbox_distances(bb))
+    np.testing.assert_array_equal(d[:, 0], bb[:, 1] - bb[:, 0])
+    np.testing.assert_array_equal(d[:, 1], bb[:, 3] - bb[:, 2])
+    np.testing.assert_allclose(d[:, 2], np.hypot(d[:, 0], d[:, 1]), rtol=4e-16, atol=0)
+    assert (d[:, 2] >= np.maximum(d[:, 0], d[:, 1])).all()
+
+
+def check_default_tolerance_of_tree(impl):
+    """The default point tolerance is 1e-12 x the LARGEST cell diagonal: a unit square next to
+    a 1000 x 1 strip gives tol = 1e-12 * sqrt(1000^2 + 1) ~ 1.0000005e-9, so a point 5e-10
+    outside the square's left edge is on it (a tolerance from the smallest diagonal, 1.4e-12,
+    would miss it) and a point 2e-9 outside is not (a diagonal without the sqrt, 1e6, would
+    give 1e-6 and find it)."""
+    v = np.array([[0.0, 0.0], [1.0, 0.0], [1.0, 1.0], [0.0, 1.0], [1001.0, 0.0], [1001.0, 1.0]])
+    f = np.array([[0, 1, 2, 3], [1, 4, 5, 2]])
+    tree = impl.CellTree2d(v, f, -1)
+    pts = np.array([[-5e-10, 0.5], [-2e-9, 0.5], [0.5, -5e-10], [0.5, 1.0 + 2e-9]])
+    np.testing.assert_array_equal(tree.locate_points(pts), [0, -1, 0, -1])
+
+
+UTIL_CASES = [
+    check_default_tolerance, check_bbox_tree, check_bbox_distances, check_bbox_distances_pythagoras,
+    check_default_tolerance_of_tree,
+]

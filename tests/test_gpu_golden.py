@@ -28,10 +28,13 @@ def impl():
         build_face_bboxes=A.build_face_bboxes,
         build_edge_bboxes=A.build_edge_bboxes,
         sort_intersections_by_edge=A.sort_intersections_by_edge,
+        bbox_tree=A.bbox_tree,
+        bbox_distances=A.bbox_distances,
+        default_tolerance=A.default_tolerance,
     )
 
 
-@pytest.mark.parametrize("case", C.TREE_CASES + C.MODULE_CASES, ids=lambda f: f.__name__)
+@pytest.mark.parametrize("case", C.TREE_CASES + C.MODULE_CASES + C.UTIL_CASES, ids=lambda f: f.__name__)
 def test_reference_case(impl, case):
     print(f"[gpu golden] {case.__name__}", file=sys.stderr, flush=True)
     case(impl)

@@ -286,3 +286,47 @@ def test_barycentric_adjoint():
         lambda a, b: BarycentricTriangleFunction.apply(a, b, tree.d_elements, face[:50], tol), (small, vs),
         eps=1e-6, atol=1e-8, rtol=1e-6,
     )
+
+
+def test_inputs_are_checked_or_converted():
+    """float32 / host inputs are converted to float64 on the GPU by Regridder and
+    barycentric_weights (same result as float64 device inputs); the autograd Functions
+    called directly reject them, and wrong sizes raise before any kernel runs."""
+    import torch
+
+    import numba_celltree_amd as M
+    from numba_celltree_amd.barycentric import BarycentricTriangleFunction, barycentric_weights
+    from numba_celltree_amd.regrid import RegridFunction
+
+    sv, sf, tv, tf, F = problem(21, 8, 4.0, n_fields=3)
+    tree, rg = build(sv, sf, tv, tf)
+    F64 = torch.from_numpy(F).cuda()
+    want = rg(F64)
+    F32 = F64.float()
+    assert torch.equal(rg(F32), rg(F32.double()))
+    assert torch.equal(rg(F), want)  # numpy in
+    assert torch.equal(rg(F64.cpu()), want)  # host tensor in
+    topo = rg.topology
+    with pytest.raises(TypeError):
+        RegridFunction.apply(F32, tree.d_vertices, topo.tgt_vertices, topo)
+    with pytest.raises(ValueError):
+        RegridFunction.apply(F64.cpu(), tree.d_vertices, topo.tgt_vertices, topo)
+    with pytest.raises(ValueError):
+        rg(F64[:-1])
+    with pytest.raises(ValueError):
+        rg(F64, src_vertices=tree.d_vertices[:-1])
+    with pytest.raises(ValueError):
+        rg(F64, tgt_vertices=topo.tgt_vertices[:, :1])
+
+    v, f, _ = S.config1()
+    tri = M.CellTree2d(v, f, -1)
+    pts = torch.from_numpy(np.random.default_rng(2).uniform(0.5, 9.5, (1000, 2))).cuda()
+    face, w = barycentric_weights(tri, pts)
+    face32, w32 = barycentric_weights(tri, pts.float())
+    face_h, w_h = barycentric_weights(tri, pts.cpu().numpy())
+    assert torch.equal(face_h, face) and torch.equal(w_h, w)
+    assert torch.equal(w32, barycentric_weights(tri, pts.float().double())[1])
+    with pytest.raises(ValueError):
+        barycentric_weights(tri, pts, vertices=tri.d_vertices[:-1])
+    with pytest.raises(TypeError):
+        BarycentricTriangleFunction.apply(pts.float(), tri.d_vertices, tri.d_elements, face, 1e-12)

@@ -132,3 +132,23 @@ def test_start_grid_size_rule():
     assert CB.start_grid_size(1024) == 32
     assert CB.start_grid_size(1025) == 64
     assert CB.start_grid_size(1_201_919) == CB.START_GRID_MAX == 1024
+
+
+def test_element_indices_are_checked_before_any_launch():
+    """Out-of-range vertex indices, and -1 in edge rows (which have no fill value), raise
+    ValueError on the host: the kernels read vertices without bounds checks."""
+    from numba_celltree_amd import _device as D
+
+    faces = np.array([[0, 1, 2], [1, 3, 2]])
+    with pytest.raises(ValueError, match="out of bounds"):
+        D.faces_to_dev(faces, 3)
+    with pytest.raises(ValueError, match=r"\[-1, n_vertices\)"):
+        D.faces_to_dev(np.array([[0, 1, -2]]), 4)
+    with pytest.raises(ValueError, match=r"\[0, n_vertices\)"):
+        D.faces_to_dev(np.array([[0, 1], [1, -1]]), 4, allow_fill=False)
+    with pytest.raises(ValueError, match="indices must lie"):
+        D.index_to_dev(np.array([0, 5]), 5)
+    with pytest.raises(ValueError, match="indices must lie"):
+        D.index_to_dev(np.array([0, -1]), 5)
+    with pytest.raises(ValueError, match="indices must lie"):
+        D.index_to_dev(np.array([0, -2]), 5, allow_missing=True)

@@ -26,10 +26,13 @@ IMPL = types.SimpleNamespace(
     build_face_bboxes=R.build_face_bboxes,
     build_edge_bboxes=R.build_edge_bboxes,
     sort_intersections_by_edge=R.sort_intersections_by_edge,
+    bbox_tree=R.bbox_tree,
+    bbox_distances=R.bbox_distances,
+    default_tolerance=R.default_tolerance,
 )
 
 
-@pytest.mark.parametrize("case", C.TREE_CASES + C.MODULE_CASES, ids=lambda f: f.__name__)
+@pytest.mark.parametrize("case", C.TREE_CASES + C.MODULE_CASES + C.UTIL_CASES, ids=lambda f: f.__name__)
 def test_reference_case(case):
     case(IMPL)
 
@@ -306,3 +309,33 @@ def test_point_in_polygon_or_on_edge():
     assert not R.point_in_polygon_or_on_edge([0.5, -1e-8], tri, 1e-9)
     assert R.point_on_edge([0.5, 1e-10], [0.0, 0.0, 1.0, 0.0], 1e-9)
     assert not R.point_on_edge([1.5, 0.0], [0.0, 0.0, 1.0, 0.0], 1e-9)
+
+
+def _no_sqrt_distances(bb):
+    d = np.empty((bb.shape[0], 3))
+    d[:, 0] = bb[:, 1] - bb[:, 0]
+    d[:, 1] = bb[:, 3] - bb[:, 2]
+    d[:, 2] = d[:, 0] ** 2 + d[:, 1] ** 2
+    return d
+
+
+def _min_tolerance(bb_diagonal):
+    return max(C.MIN_TOLERANCE, C.TOLERANCE_FACTOR * min(bb_diagonal))
+
+
+@pytest.mark.parametrize(
+    "attr, planted", [("bbox_distances", _no_sqrt_distances), ("default_tolerance", _min_tolerance)]
+)
+def test_util_cases_catch_planted_slips(monkeypatch, attr, planted):
+    """A dropped sqrt in bbox_distances, or min for max in default_tolerance, planted in the
+    oracle, fails the utility cases (so those functions are pinned, not just exercised)."""
+    monkeypatch.setattr(R, attr, planted)
+    impl = types.SimpleNamespace(**{**vars(IMPL), attr: planted})
+    failed = []
+    for case in C.UTIL_CASES:
+        try:
+            case(impl)
+        except AssertionError:
+            failed.append(case.__name__)
+    assert failed, f"no utility case noticed the planted {attr}"
+    assert "check_default_tolerance_of_tree" in failed
