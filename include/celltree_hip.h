@@ -180,6 +180,20 @@ int ct_locate_points_in_order(const ct_tree* tree, const double* points, int64_t
                               const int32_t* order, const double* sorted, double tol, int64_t* out,
                               void* stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The same results
+ * as ct_locate_points_ordered (query.py:110-117 locate_points), with the ordering workspace
+ * taken from a static device array inside the library instead of an argument: no hipMalloc,
+ * hipFree or other allocation happens in the call.  n is at most CT_NOALLOC_MAX_POINTS (larger
+ * n returns CT_ERR_ARG).  Each device keeps its own array (about 36 MiB) for the life of the
+ * process, resolved on the first call on that device (make that first call outside graph
+ * capture), and shared by every call on it: calls that overlap on one device race and must
+ * be serialised by the caller (one stream, or events).  `stream` must belong to the current
+ * device. */
+#define CT_NOALLOC_MAX_POINTS (1 << 21)
+int ct_locate_points_ordered_noalloc(const ct_tree* tree, const double* points, int64_t n, double tol,
+                                     int64_t* out, void* stream);
+
 /* query.py:167-174 locate_points_on_edge (EdgeCellTree2d). */
 int ct_locate_points_on_edge(const ct_tree* tree, const double* points, int64_t n, double tol,
                              int64_t* out, void* stream);

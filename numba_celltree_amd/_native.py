@@ -19,6 +19,7 @@ CT_ERR_DEPTH = -4
 CT_ERR_WORKSPACE = -5
 CT_MAX_N_VERTEX = 32
 CT_MAX_DEPTH = 254
+CT_NOALLOC_MAX_POINTS = 1 << 21
 
 
 class CtNode(ctypes.Structure):
@@ -149,6 +150,7 @@ SIGNATURES = {
     "ct_locate_workspace_bytes": (_SZ, [_I64]),
     "ct_locate_points_ordered": (ctypes.c_int, [_TP, _P, _I64, _F64, _P, _SZ, _P, _P]),
     "ct_locate_points_in_order": (ctypes.c_int, [_TP, _P, _I64, _P, _P, _F64, _P, _P]),
+    "ct_locate_points_ordered_noalloc": (ctypes.c_int, [_TP, _P, _I64, _F64, _P, _P]),
     "ct_locate_points_on_edge": (ctypes.c_int, [_TP, _P, _I64, _F64, _P, _P]),
     "ct_order_workspace_bytes": (_SZ, [_I64]),
     "ct_query_order": (ctypes.c_int, [_I32, _P, _I64, ctypes.POINTER(_F64), _P, _SZ, _P, _P, _P]),

@@ -4,6 +4,8 @@
 // (ct_exclusive_scan), and a write pass at the scanned offsets -- no atomic append; output
 // order = query order, then DFS order.  Queries may be processed in any lane order
 // (`order`, from ct_query_order): every query writes only its own count / rows.
+#include <mutex>
+
 #include "ct_internal.h"
 #include "ct_primitives.h"
 #include "ct_traverse.h"
@@ -314,7 +316,9 @@ struct OrderWs {
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
-OrderWs carve_order(char* base, int64_t n, bool with_order, size_t* total) {
+// with_rows: also the query rows in coarse-bin order (cq) and the sorted queries, which a
+// caller needs only when it asks query_order for `sorted`.
+OrderWs carve_order(char* base, int64_t n, bool with_order, size_t* total, bool with_rows = true) {
   size_t off = 0;
   OrderWs w{};
   auto take = [&](size_t bytes) {
@@ -332,9 +336,9 @@ OrderWs carve_order(char* base, int64_t n, bool with_order, size_t* total) {
   w.kraw = reinterpret_cast<unsigned*>(take(sizeof(unsigned) * nn));
   w.keys = reinterpret_cast<unsigned*>(take(sizeof(unsigned) * nn));
   w.idx = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * nn));
-  w.cq = reinterpret_cast<double*>(take(sizeof(double) * 4 * nn));
+  w.cq = with_rows ? reinterpret_cast<double*>(take(sizeof(double) * 4 * nn)) : nullptr;
   w.order = with_order ? reinterpret_cast<int32_t*>(take(sizeof(int32_t) * nn)) : nullptr;
-  w.sorted = with_order ? reinterpret_cast<double*>(take(sizeof(double) * 2 * nn)) : nullptr;
+  w.sorted = with_order && with_rows ? reinterpret_cast<double*>(take(sizeof(double) * 2 * nn)) : nullptr;
   *total = off + 256;
   return w;
 }
@@ -928,7 +932,70 @@ int launch_bary(const double* points, const int64_t* fi, int64_t n, const int32_
 
 using namespace ct;
 
+// ------------------------------------------------------------ allocation-free locate
+// The workspace of ct_locate_points_ordered_noalloc: the order kernels' keys, indices, tile
+// histograms and the order itself, 16 bytes per point plus about 1 MiB at the maximum of
+// CT_NOALLOC_MAX_POINTS points (checked on every call).  Each device holds its own copy for
+// the life of the process, and every call on that device reuses it: concurrent calls on one
+// device race on it.  Nothing in it needs clearing (every byte a call reads, the same call
+// writes first).
+static constexpr size_t kNoAllocScratchBytes = size_t(36) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// The device address of the current device's `scratch`, resolved once per device (the host
+// name `scratch` is only a stand-in and must never reach a kernel).
+int scratch_address(void** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    set_error("hipGetDevice: %s", hipGetErrorString(e));
+    return CT_ERR_HIP;
+  }
+  CT_REQUIRE(dev >= 0 && dev < kNoAllocMaxDevices, "device %d beyond the %d supported", dev, kNoAllocMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      set_error("hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+      return CT_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return CT_OK;
+}
+}  // namespace
+
 extern "C" {
+
+int ct_locate_points_ordered_noalloc(const ct_tree* tree, const double* points, int64_t n, double tol,
+                                     int64_t* out, void* stream) {
+  int rc = check_tree(tree);
+  if (rc) return rc;
+  CT_REQUIRE(n >= 0 && tree->n_max_vert >= 3 && (n == 0 || (points && out)), "bad arguments");
+  CT_REQUIRE(n <= CT_NOALLOC_MAX_POINTS, "%lld points exceed the allocation-free maximum of %d",
+             (long long)n, CT_NOALLOC_MAX_POINTS);
+  if (n == 0) return CT_OK;
+  void* ws = nullptr;
+  rc = scratch_address(&ws);
+  if (rc) return rc;
+  size_t need = 0;
+  const OrderWs w = carve_order(static_cast<char*>(ws), n, true, &need, /*with_rows=*/false);
+  if (need > kNoAllocScratchBytes) {
+    set_error("allocation-free workspace too small: %zu < %zu", kNoAllocScratchBytes, need);
+    return CT_ERR_WORKSPACE;
+  }
+  hipStream_t st = as_stream(stream);
+  rc = query_order(0, points, n, tree->bbox, w, w.order, nullptr, st);
+  if (rc) return rc;
+  return launch_locate_faces(tree, points, QueryOrder{w.order, nullptr}, n, tol, out, st);
+}
 
 int ct_locate_points(const ct_tree* tree, const double* points, int64_t n, double tol, int64_t* out,
                      void* stream) {
