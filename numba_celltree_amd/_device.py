@@ -67,8 +67,9 @@ def faces_to_dev(faces, n_vertices: int, allow_fill: bool = True) -> torch.Tenso
     """int32 device copy of a (n, n_max_vert) index array.
 
     Every index must address one of ``n_vertices`` vertices; -1 (the fill value) is allowed
-    only where ``allow_fill`` (face rows; edge rows have no fill).  The kernels read vertices
-    without bounds checks, so this is checked here, before any launch."""
+    only where ``allow_fill`` (face rows, past their first 3 entries; edge rows have no
+    fill).  The kernels read vertices without bounds checks, so this is checked here, before
+    any launch."""
     if isinstance(faces, torch.Tensor):
         f = faces
     else:
@@ -80,6 +81,9 @@ def faces_to_dev(faces, n_vertices: int, allow_fill: bool = True) -> torch.Tenso
         if lo < (-1 if allow_fill else 0):
             what = "[-1, n_vertices)" if allow_fill else "[0, n_vertices)"
             raise ValueError(f"element indices must lie in {what}, found {lo}")
+        if allow_fill and f.ndim == 2 and f.shape[1] >= 3 and int(f[:, :3].min()) < 0:
+            # polygon_length (geometry_utils.py:72-79) takes the first 3 entries as vertices
+            raise ValueError("every face needs at least 3 vertices (a fill value among its first 3 entries)")
     return f.to(device=device(), dtype=torch.int32).contiguous()
 
 

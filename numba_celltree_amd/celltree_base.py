@@ -76,6 +76,16 @@ class CellTree2dBase:
             D.ptr(nodes_raw), D.ptr(bb_indices), ctypes.byref(n_nodes), ctypes.byref(depth), st,
         )
         del ws
+        self._adopt(vertices, elements, bb_coords, nodes_raw[: 32 * n_nodes.value], bb_indices,
+                    int(n_nodes.value), int(depth.value), n_buckets, cells_per_leaf)
+
+    def _adopt(self, vertices: torch.Tensor, elements: torch.Tensor, bb_coords: torch.Tensor,
+               nodes_raw: torch.Tensor, bb_indices: torch.Tensor, n_nodes: int, depth: int,
+               n_buckets: int, cells_per_leaf: int) -> None:
+        """Takes a built (or imported) tree's device arrays and derives the rest: the bbox
+        statistics, the ct_tree descriptor and the device-only layouts."""
+        n = int(elements.shape[0])
+        st = D.stream()
         distances = D.empty((n, 3), torch.float64)
         stats = D.empty(5, torch.float64)
         _native.call("ct_bbox_stats", D.ptr(bb_coords), n, D.ptr(distances), D.ptr(stats), st)
@@ -86,11 +96,11 @@ class CellTree2dBase:
         self.d_vertices = vertices
         self.d_elements = elements
         self.d_bb_coords = bb_coords
-        self.d_nodes = nodes_raw[: 32 * n_nodes.value]
+        self.d_nodes = nodes_raw
         self.d_bb_indices = bb_indices
         self.d_bb_distances = distances
-        self.n_nodes = int(n_nodes.value)
-        self.depth = int(depth.value)
+        self.n_nodes = int(n_nodes)
+        self.depth = int(depth)
         self._bbox = host[:4].copy()
         self._max_diagonal = float(host[4])
         t = _native.CtTree()
@@ -142,6 +152,89 @@ class CellTree2dBase:
             _native.call("ct_node_boxes", self.tree_ref, D.ptr(scratch), D.ptr(self.d_node_boxes), st)
             del scratch
             t.node_boxes = self.d_node_boxes.data_ptr()
+
+    # ------------------------------------------------------------ export / import
+    KIND = "faces"
+
+    def to_arrays(self) -> dict:
+        """The tree as host arrays in the reference's layouts (see numba_celltree_amd.treeio):
+        vertices, elements (CCW faces or edges), nodes (NodeDType), bb_indices, bb_coords,
+        n_buckets, cells_per_leaf and kind.  Copies from the device."""
+        from numba_celltree_amd import _device as _D
+
+        return {
+            "vertices": self.d_vertices.cpu().numpy(),
+            "elements": self._elements_host().copy(),
+            "nodes": _D.nodes_to_host(self.d_nodes, self.n_nodes),
+            "bb_indices": self.d_bb_indices.cpu().numpy().astype(np.intp),
+            "bb_coords": self.d_bb_coords.cpu().numpy(),
+            "n_buckets": int(self.n_buckets),
+            "cells_per_leaf": int(self.cells_per_leaf),
+            "kind": self.KIND,
+        }
+
+    @classmethod
+    def from_arrays(cls, vertices, elements, nodes, bb_indices, bb_coords, n_buckets: int = 4,
+                    cells_per_leaf: int = 2, kind: Optional[str] = None, **_ignored):
+        """A tree from exported arrays (``to_arrays``, ``load``, or a reference tree's own
+        attributes), without rebuilding it.  The arrays are validated on the host; the device
+        layouts are derived again."""
+        from numba_celltree_amd import treeio
+
+        arrays = treeio.validate({
+            "vertices": vertices, "elements": elements, "nodes": nodes, "bb_indices": bb_indices,
+            "bb_coords": bb_coords, "n_buckets": n_buckets, "cells_per_leaf": cells_per_leaf,
+            "kind": cls.KIND if kind is None else kind,
+        })
+        if arrays["kind"] != cls.KIND:
+            raise ValueError(f"{cls.__name__} holds a {cls.KIND} tree, the arrays are a {arrays['kind']} tree")
+        tree = cls.__new__(cls)
+        tree._import(arrays)
+        return tree
+
+    def _import(self, a: dict) -> None:
+        d_vertices = D.to_dev(a["vertices"], torch.float64)
+        allow_fill = a["kind"] == "faces"
+        d_elements = D.faces_to_dev(a["elements"], d_vertices.shape[0], allow_fill=allow_fill)
+        d_bb = D.to_dev(a["bb_coords"], torch.float64)
+        # bb_coords must be the elements' own bounding boxes: queries prune with them.
+        n, nmax = int(d_elements.shape[0]), int(d_elements.shape[1])
+        check = D.empty((n, 4), torch.float64)
+        st = D.stream()
+        if allow_fill:
+            _native.call("ct_build_face_bboxes", D.ptr(d_elements), n, nmax, D.ptr(d_vertices), D.ptr(check), st)
+        else:
+            _native.call("ct_build_edge_bboxes", D.ptr(d_elements), n, D.ptr(d_vertices), self._edge_tolerance(d_vertices),
+                         D.ptr(check), st)
+        same = (check == d_bb) | (torch.isnan(check) & torch.isnan(d_bb))
+        if not bool(same.all()):
+            raise ValueError("bb_coords are not the bounding boxes of the elements")
+        del check
+        d_nodes = D.nodes_to_device(a["nodes"])
+        d_bb_indices = D.to_dev(a["bb_indices"], torch.int32)
+        self._adopt(d_vertices, d_elements, d_bb, d_nodes, d_bb_indices, len(a["nodes"]), a["depth"],
+                    a["n_buckets"], a["cells_per_leaf"])
+
+    def save(self, file) -> None:
+        """Writes the tree to ``file`` (a path or file object) as an .npz archive."""
+        from numba_celltree_amd import treeio
+
+        treeio.save(file, self.to_arrays())
+
+    @classmethod
+    def load(cls, file):
+        """Reads a tree written by ``save`` (validated; not rebuilt)."""
+        from numba_celltree_amd import treeio
+
+        return cls.from_arrays(**treeio.load(file))
+
+    def __getstate__(self):
+        return self.to_arrays()
+
+    def __setstate__(self, state):
+        from numba_celltree_amd import treeio
+
+        self._import(treeio.validate(state))
 
     @property
     def tree_ref(self):

@@ -30,16 +30,21 @@ class EdgeCellTree2d(CellTree2dBase):
         if d_edges.ndim != 2 or d_edges.shape[1] != 2:
             raise ValueError("edges must have shape (n_edge, 2)")
         st = D.stream()
-        ext = D.empty(4, torch.float64)
-        _native.call("ct_vertex_extent", D.ptr(d_vertices), int(d_vertices.shape[0]), D.ptr(ext), st)
-        xmin, xmax, ymin, ymax = ext.cpu().numpy().tolist()
-        dx = xmax - xmin
-        dy = ymax - ymin
-        tol = max(MIN_TOLERANCE, TOLERANCE_FACTOR * max(dx, dy))
+        tol = self._edge_tolerance(d_vertices)
         n = int(d_edges.shape[0])
         bb = D.empty((n, 4), torch.float64)
         _native.call("ct_build_edge_bboxes", D.ptr(d_edges), n, D.ptr(d_vertices), tol, D.ptr(bb), st)
         self._build_tree(d_vertices, d_edges, bb, n_buckets, cells_per_leaf)
+
+    KIND = "edges"
+
+    @staticmethod
+    def _edge_tolerance(d_vertices: torch.Tensor) -> float:
+        """max(1e-15, 1e-12 * the larger extent of the vertices) (edge_celltree.py:60-64)."""
+        ext = D.empty(4, torch.float64)
+        _native.call("ct_vertex_extent", D.ptr(d_vertices), int(d_vertices.shape[0]), D.ptr(ext), D.stream())
+        xmin, xmax, ymin, ymax = ext.cpu().numpy().tolist()
+        return max(MIN_TOLERANCE, TOLERANCE_FACTOR * max(xmax - xmin, ymax - ymin))
 
     @property
     def edges(self) -> np.ndarray:

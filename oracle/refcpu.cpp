@@ -68,20 +68,35 @@ inline int polygon_length(const int64_t* face, int n) {
   return n;
 }
 
+// FP64 operation count of the executed path, for the roofline of the clip kernels: every
+// add, subtract, multiply and divide of separating_axes, polygon_polygon_clip_area and
+// polygon_area (compares, abs, min / max and sign flips are not counted).  Functions take
+// COUNT = false by default, which compiles the counting out; ref_face_pair_flops runs them
+// with COUNT = true, one pair at a time per thread.
+thread_local int64_t t_flops = 0;
+template <bool COUNT>
+inline void flops(int64_t k) {
+  if constexpr (COUNT) t_flops += k;
+}
+
 // geometry_utils.py:82-95 (fan triangulation, abs per triangle)
+template <bool COUNT = false>
 double polygon_area(const Point* poly, int length) {
   double area = 0.0;
   Point a = poly[0];
   Point b = poly[1];
   Vector U = to_vector(a, b);
+  flops<COUNT>(2);
   for (int i = 2; i < length; ++i) {
     Point c = poly[i];
     Vector V = to_vector(c, a);
     area += std::fabs(cross_product(U, V));
+    flops<COUNT>(2 + 3 + 1);  // V, the cross product, the sum
     b = c;
     U = V;
   }
   (void)b;
+  flops<COUNT>(1);
   return 0.5 * area;
 }
 
@@ -457,7 +472,9 @@ bool cyrus_beck_line_polygon_clip(Point a, Point b, const Point* poly, int lengt
 // ---------------------------------------------------------------------------
 
 // separating_axis.py:17-27
+template <bool COUNT = false>
 inline void extrema_projected(Vector norm, const Point* poly, int length, double* mn, double* mx) {
+  flops<COUNT>(3 * (int64_t)length);  // one dot product per vertex
   double min_proj = FLOAT_MAX, max_proj = FLOAT_MIN;
   for (int i = 0; i < length; ++i) {
     double proj = dot_product({poly[i].x, poly[i].y}, norm);
@@ -469,22 +486,25 @@ inline void extrema_projected(Vector norm, const Point* poly, int length, double
 }
 
 // separating_axis.py:30-39
+template <bool COUNT = false>
 inline bool is_separating_axis(Vector norm, const Point* a, const Point* b, int la, int lb) {
   double mina, maxa, minb, maxb;
-  extrema_projected(norm, a, la, &mina, &maxa);
-  extrema_projected(norm, b, lb, &minb, &maxb);
+  extrema_projected<COUNT>(norm, a, la, &mina, &maxa);
+  extrema_projected<COUNT>(norm, b, lb, &minb, &maxb);
   return !(maxa > minb && maxb > mina);
 }
 
 // separating_axis.py:42-55
+template <bool COUNT = false>
 bool separating_axes(const Point* a, int la, const Point* b, int lb) {
   Point p = a[la - 1];
   for (int i = 0; i < la; ++i) {
     Point q = a[i];
     Vector norm = {p.y - q.y, q.x - p.x};
+    flops<COUNT>(2);
     p = q;
     if (norm.x == 0.0 && norm.y == 0.0) continue;
-    if (is_separating_axis(norm, a, b, la, lb)) return false;
+    if (is_separating_axis<COUNT>(norm, a, b, la, lb)) return false;
   }
   return true;
 }
@@ -494,16 +514,21 @@ bool separating_axes(const Point* a, int la, const Point* b, int lb) {
 // ---------------------------------------------------------------------------
 
 // sutherland_hodgman.py:56-60 (strict)
+template <bool COUNT = false>
 inline bool sh_inside(Point p, Point r, Vector U) {
+  flops<COUNT>(4);
   return U.x * (p.y - r.y) > U.y * (p.x - r.x);
 }
 
 // sutherland_hodgman.py:63-74
+template <bool COUNT = false>
 inline bool sh_intersection(Point a, Vector V, Point r, Vector N, Point* out) {
   Vector W = {r.x - a.x, r.y - a.y};
   double nw = dot_product(N, W);
   double nv = dot_product(N, V);
+  flops<COUNT>(2 + 3 + 3);
   if (nv != 0) {
+    flops<COUNT>(1 + 4);  // t, then the point
     double t = nw / nv;
     *out = {a.x + t * V.x, a.y + t * V.y};
     return true;
@@ -513,6 +538,7 @@ inline bool sh_intersection(Point a, Vector V, Point r, Vector N, Point* out) {
 }
 
 // sutherland_hodgman.py:84-148.  subject = polygon, clipper = clipper.
+template <bool COUNT = false>
 double polygon_polygon_clip_area(const Point* polygon, int n_polygon, const Point* clipper,
                                  int n_clip) {
   Point subject[CLIP_MAX_N_VERTEX];
@@ -523,28 +549,30 @@ double polygon_polygon_clip_area(const Point* polygon, int n_polygon, const Poin
   for (int i = 0; i < n_clip; ++i) {
     Point s = clipper[i];
     Vector U = {s.x - r.x, s.y - r.y};
+    flops<COUNT>(2);
     if (U.x == 0 && U.y == 0) continue;
     Vector N = {-U.y, U.x};
     int length = n_output;
     for (int j = 0; j < length; ++j) subject[j] = output[j];
     n_output = 0;
     Point a = subject[length - 1];
-    bool a_inside = sh_inside(a, r, U);
+    bool a_inside = sh_inside<COUNT>(a, r, U);
     for (int j = 0; j < length; ++j) {
       Point b = subject[j];
       Vector V = {b.x - a.x, b.y - a.y};
+      flops<COUNT>(2);
       if (V.x == 0 && V.y == 0) continue;
-      bool b_inside = sh_inside(b, r, U);
+      bool b_inside = sh_inside<COUNT>(b, r, U);
       if (b_inside) {
         if (!a_inside) {
           Point pt;
-          if (sh_intersection(a, V, r, N, &pt) && n_output < CLIP_MAX_N_VERTEX)
+          if (sh_intersection<COUNT>(a, V, r, N, &pt) && n_output < CLIP_MAX_N_VERTEX)
             output[n_output++] = pt;
         }
         if (n_output < CLIP_MAX_N_VERTEX) output[n_output++] = b;
       } else if (a_inside) {
         Point pt;
-        if (sh_intersection(a, V, r, N, &pt)) {
+        if (sh_intersection<COUNT>(a, V, r, N, &pt)) {
           if (n_output < CLIP_MAX_N_VERTEX) output[n_output++] = pt;
         } else {
           b_inside = true;
@@ -557,7 +585,7 @@ double polygon_polygon_clip_area(const Point* polygon, int n_polygon, const Poin
     if (n_output < 3) return 0.0;
     r = s;
   }
-  return polygon_area(output, n_output);
+  return polygon_area<COUNT>(output, n_output);
 }
 
 // geometry_utils.py:513-524
@@ -1287,6 +1315,30 @@ void ref_area_of_intersection(const double* va, const double* vb, const int64_t*
     int la = copy_vertices(va, fa + ia[i] * na_max, na_max, a);
     int lb = copy_vertices(vb, fb + ib[i] * nb_max, nb_max, b);
     out[i] = polygon_polygon_clip_area(a, la, b, lb);
+  }
+}
+
+// Operation counts of intersect_faces' pair stage (celltree.py:248-269) for every candidate
+// pair: sat[i] = FP64 ops of polygons_intersect (separating_axis.py:58-75, both directions,
+// short-circuit as `and`), passed[i] = its result, clip[i] = ops of area_of_intersection
+// (sutherland_hodgman.py:151-168, clip + polygon_area) for pairs that passed, else 0, and
+// area[i] = that area (0 where not computed).  Counts are of the executed path (see
+// flops<>).
+void ref_face_pair_flops(const double* va, const double* vb, const int64_t* fa, int na_max,
+                         const int64_t* fb, int nb_max, const int64_t* ia, const int64_t* ib,
+                         int64_t n, int64_t* sat, uint8_t* passed, int64_t* clip, double* area) {
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int64_t i = 0; i < n; ++i) {
+    Point a[MAX_N_VERTEX], b[MAX_N_VERTEX];
+    int la = copy_vertices(va, fa + ia[i] * na_max, na_max, a);
+    int lb = copy_vertices(vb, fb + ib[i] * nb_max, nb_max, b);
+    t_flops = 0;
+    const bool ok = separating_axes<true>(a, la, b, lb) && separating_axes<true>(b, lb, a, la);
+    sat[i] = t_flops;
+    passed[i] = ok;
+    t_flops = 0;
+    area[i] = ok ? polygon_polygon_clip_area<true>(a, la, b, lb) : 0.0;
+    clip[i] = t_flops;
   }
 }
 

@@ -106,6 +106,7 @@ def lib():
             "ref_pairs_free": (None, [P]),
             "ref_polygons_intersect": (None, [P, P, P, i32, P, i32, P, P, i64, P]),
             "ref_area_of_intersection": (None, [P, P, P, i32, P, i32, P, P, i64, P]),
+            "ref_face_pair_flops": (None, [P, P, P, i32, P, i32, P, P, i64, P, P, P, P]),
             "ref_box_area_of_intersection": (None, [P, P, P, i32, P, P, i64, P]),
             "ref_barycentric_triangle_weights": (None, [P, P, i64, P, i32, P, f64, P]),
             "ref_barycentric_wachspress_weights": (None, [P, P, i64, P, i32, P, f64, P]),
@@ -265,6 +266,27 @@ def area_of_intersection(vertices_a, vertices_b, faces_a, faces_b, indices_a, in
         _p(ia), _p(ib), len(ia), _p(out),
     )
     return out
+
+
+def face_pair_flops(vertices_a, vertices_b, faces_a, faces_b, indices_a, indices_b):
+    """Per candidate pair of intersect_faces' pair stage: (sat_ops, passed, clip_ops, area).
+
+    sat_ops counts the FP64 adds, subs, muls and divs polygons_intersect executes
+    (separating_axis.py:58-75, both directions, short-circuit), clip_ops those of
+    area_of_intersection (sutherland_hodgman.py:84-168 plus polygon_area) on pairs that
+    passed (0 otherwise); area is that area (0 where not computed)."""
+    fa, fb = _i64(faces_a), _i64(faces_b)
+    ia, ib = _i64(indices_a), _i64(indices_b)
+    n = len(ia)
+    sat = np.empty(n, dtype=np.int64)
+    passed = np.empty(n, dtype=np.uint8)
+    clip = np.empty(n, dtype=np.int64)
+    area = np.empty(n, dtype=np.float64)
+    lib().ref_face_pair_flops(
+        _p(_f64(vertices_a)), _p(_f64(vertices_b)), _p(fa), fa.shape[1], _p(fb), fb.shape[1],
+        _p(ia), _p(ib), n, _p(sat), _p(passed), _p(clip), _p(area),
+    )
+    return sat, passed.astype(bool), clip, area
 
 
 def box_area_of_intersection(bbox_coords, vertices, faces, indices_bbox, indices_face):
