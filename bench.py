@@ -67,34 +67,70 @@ def time_calls(fn, steps: int):
     return wall, float(np.mean(per))
 
 
-def cpu_baseline(v, f, pts, min_seconds: float = 10.0, max_reps: int = 3):
+def oracle_native():
+    """The CPU oracle at the highest ISA level this host executes (same results at every
+    level), and a description of the host for the baseline records."""
     from oracle import refcpu as R
 
-    tree = R.OracleCellTree2d(v, f, -1)
-    reps, total, t_all = 0, 0, 0.0
-    while reps < max_reps and t_all < min_seconds:
+    level = R.native_isa_level()
+    R.use_isa_level(level)
+    model, _ = R.host_cpu()
+    return R, {"cores": int(R.lib().ref_num_threads()), "cpu": model, "isa": level}
+
+
+def median_time(fn, runs: int = 5):
+    """One warm-up call, then the median wall time of `runs` calls (seconds)."""
+    fn()
+    ts = []
+    for _ in range(runs):
         t = time.perf_counter()
-        tree.locate_points(pts)
-        t_all += time.perf_counter() - t
-        total += len(pts)
-        reps += 1
+        fn()
+        ts.append(time.perf_counter() - t)
+    return float(np.median(ts)), ts
+
+
+def cpu_baseline(v, f, pts, runs: int = 5):
+    R, host = oracle_native()
+    tree = R.OracleCellTree2d(v, f, -1)
+    t, ts = median_time(lambda: tree.locate_points(pts), runs)
     return {
-        "value": total / t_all,
+        "value": len(pts) / t,
         "unit": "points/s",
-        "cores": int(R.lib().ref_num_threads()),
+        **host,
         "kind": "port",
-        "sample": f"{reps} x {len(pts):,} points of the same workload (oracle tree of the same mesh)",
+        "sample": f"the whole workload ({len(pts):,} points, oracle tree of the same mesh): one warm-up, "
+                  f"median of {runs} runs ({min(ts):.3f}-{max(ts):.3f} s)",
+    }
+
+
+def flop_counts():
+    """The committed FP64 operation counts of the overlap workloads (oracle-counted)."""
+    with open(os.path.join(ROOT, "tests", "golden", "flops_overlap.json")) as fh:
+        return json.load(fh)
+
+
+def fp64_roofline(kernel: str, flops: int, kernel_ms: float, note: str):
+    achieved = flops / (kernel_ms * 1e-3) / 1e12
+    return {
+        "bound": "fp64", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "traffic": None, "kernel": kernel,
+        "kernel_ms": round(kernel_ms, 4), "flops": int(flops), "flops_source": note,
     }
 
 
 def extras(args):
-    """Secondary workloads, timed once each after a warm-up (device-resident inputs)."""
+    """Secondary workloads: a warm-up call each, then the mean over several calls
+    (device-resident inputs)."""
     import torch
 
     import numba_celltree_amd as M
+    from numba_celltree_amd import _device as D
+    from numba_celltree_amd import _native
     from numba_celltree_amd import synthetic as S
+    from numba_celltree_amd.cast import cast_faces
 
     out = {}
+    counts = flop_counts()
     v, f = S.config2_mesh()
     _, bx, sg = S.config2_queries(10, args.boxes, args.segments)
     tree = M.CellTree2d(torch.from_numpy(v).cuda(), torch.from_numpy(f).cuda(), -1)
@@ -103,8 +139,8 @@ def extras(args):
     for name, fn in (("locate_boxes_1e6_ms", lambda: tree.locate_boxes(d_bx)),
                      ("intersect_edges_1e6_ms", lambda: tree.intersect_edges(d_sg))):
         fn()
-        wall, _ = time_calls(fn, 3)
-        out[name] = 1e3 * wall / 3
+        wall, _ = time_calls(fn, 5)
+        out[name] = 1e3 * wall / 5
     i, _ = tree.locate_boxes(d_bx)
     out["locate_boxes_pairs"] = int(i.numel())
     i, _, _ = tree.intersect_edges(d_sg)
@@ -119,16 +155,25 @@ def extras(args):
     F = torch.from_numpy(S.sinusoid_fields(v[np.maximum(f, 0)].mean(1), 16, 1)).cuda()
     d_tv = torch.from_numpy(tv).cuda()
     d_tf = torch.from_numpy(tf).cuda()
-    t0 = time.perf_counter()
-    rg = Regridder(tree, d_tv, d_tf, -1)
-    torch.cuda.synchronize()
-    out["regrid_topology_ms"] = 1e3 * (time.perf_counter() - t0)
-    out["regrid_pairs"] = rg.topology.n_pairs
+    for label in ("regrid_topology_cold_ms", "regrid_topology_ms"):
+        t0 = time.perf_counter()
+        rg = Regridder(tree, d_tv, d_tf, -1)
+        torch.cuda.synchronize()
+        out[label] = 1e3 * (time.perf_counter() - t0)
+    topo = rg.topology
+    out["regrid_pairs"] = topo.n_pairs
     sv_t = tree.d_vertices.clone()
-    tv_t = rg.topology.tgt_vertices.clone()
-    G = torch.randn(rg.topology.n_tgt, 16, dtype=torch.float64, device="cuda")
+    tv_t = topo.tgt_vertices.clone()
+    G = torch.randn(topo.n_tgt, 16, dtype=torch.float64, device="cuda")
 
     def fwd():
+        # the full forward: the pair areas are recomputed (the area cache is dropped first)
+        topo.clear_cache()
+        with torch.no_grad():
+            return rg(F, sv_t, tv_t)
+
+    def fwd_cached():
+        # fixed vertices: the cached pair areas are reused, only the operator is applied
         with torch.no_grad():
             return rg(F, sv_t, tv_t)
 
@@ -142,37 +187,75 @@ def extras(args):
         y.backward(G)
         return Fg.grad, svg.grad, tvg.grad
 
-    for name, fn in (("regrid_fwd_16f_ms", fwd), ("regrid_fwd_bwd_16f_ms", fwd_bwd)):
+    for name, fn in (("regrid_fwd_16f_ms", fwd), ("regrid_fwd_bwd_16f_ms", fwd_bwd),
+                     ("regrid_apply_cached_16f_ms", fwd_cached)):
         fn()
         wall, _ = time_calls(fn, 20)
         out[name] = 1e3 * wall / 20
     out["regrid_fwd_bwd_over_fwd"] = out["regrid_fwd_bwd_16f_ms"] / out["regrid_fwd_16f_ms"]
-    del tree, rg
-    sv, sf, tv, tf, _ = S.config3()
     t0 = time.perf_counter()
-    src = M.CellTree2d(torch.from_numpy(sv).cuda(), torch.from_numpy(sf).cuda(), -1)
+    W = rg.weights()
     torch.cuda.synchronize()
-    out["build_999698_tris_ms"] = 1e3 * (time.perf_counter() - t0)
+    out["regrid_weights_ms"] = 1e3 * (time.perf_counter() - t0)
+    fn = lambda: W.apply(F)  # noqa: E731
+    fn()
+    wall, _ = time_calls(fn, 20)
+    out["regrid_weights_apply_16f_ms"] = 1e3 * wall / 20
+    # the pair-area kernel of the forward on its own (HIP events on the launch stream)
+    pa = lambda: topo.pair_areas(sv_t, tv_t)  # noqa: E731
+    pa()
+    _, pa_ms = time_calls(pa, 20)
+    rc = counts["regrid"]
+    if rc["kept"] == topo.n_pairs:
+        out["regrid_roofline"] = fp64_roofline(
+            "k_pair_areas (regrid forward clip)", rc["clip_flops_kept"], pa_ms,
+            "tests/golden/flops_overlap.json regrid.clip_flops_kept: Sutherland-Hodgman + polygon_area ops "
+            "of every kept pair (oracle-counted)")
+    del tree, rg, topo, W
+    sv, sf, tv, tf, _ = S.config3()
+    for label in ("build_999698_tris_cold_ms", "build_999698_tris_ms"):
+        t0 = time.perf_counter()
+        src = M.CellTree2d(torch.from_numpy(sv).cuda(), torch.from_numpy(sf).cuda(), -1)
+        torch.cuda.synchronize()
+        out[label] = 1e3 * (time.perf_counter() - t0)
     d_tv = torch.from_numpy(tv).cuda()
     d_tf = torch.from_numpy(tf).cuda()
     fn = lambda: src.intersect_faces(d_tv, d_tf, -1)  # noqa: E731
     fn()
-    wall, _ = time_calls(fn, 3)
-    out["intersect_faces_1Mx250k_ms"] = 1e3 * wall / 3
+    wall, _ = time_calls(fn, 5)
+    out["intersect_faces_1Mx250k_ms"] = 1e3 * wall / 5
     out["intersect_faces_pairs"] = int(fn()[0].numel())
-    if not args.no_cpu_baseline:
-        # CPU oracle on a bounded sample of the same workload: the first 50,000 of the
-        # 250,000 target cells against the whole source tree, scaled to all targets.
-        from oracle import refcpu as R
+    # The pair stage (fused SAT + clip + area, k_face_pairs) on the same candidate pairs,
+    # timed on its own with HIP events on the launch stream.
+    d_faces = D.faces_to_dev(cast_faces(d_tf, -1), d_tv.shape[0])
+    ii, jj = src._locate_boxes_dev(src._prepare_faces(d_tv, d_faces))
+    n = int(ii.numel())
+    area = D.empty(n, torch.float64)
+    keep = D.empty(n, torch.int64)
 
+    def pairs():
+        _native.call("ct_face_pairs", D.ptr(d_tv), D.ptr(d_faces), int(d_faces.shape[1]), D.ptr(src.d_vertices),
+                     D.ptr(src.d_elements), int(src.d_elements.shape[1]), D.ptr(ii), D.ptr(jj), n, 1, D.ptr(area),
+                     D.ptr(keep), D.stream())
+
+    pairs()
+    _, fp_ms = time_calls(pairs, 20)
+    fc = counts["intersect_faces"]
+    out["intersect_faces_candidates"] = n
+    if fc["candidates"] == n and fc["kept"] == int(keep.sum()):
+        out["intersect_faces_roofline"] = fp64_roofline(
+            "k_face_pairs (SAT + Sutherland-Hodgman + area)", fc["sat_flops"] + fc["clip_flops"], fp_ms,
+            "tests/golden/flops_overlap.json intersect_faces.sat_flops + clip_flops: the ops the reference "
+            "executes on these candidate pairs (oracle-counted)")
+    if not args.no_cpu_baseline:
+        # CPU oracle on the whole workload (all 250,000 target cells).
+        R, host = oracle_native()
         osrc = R.OracleCellTree2d(sv, sf, -1)
-        n_sample = 50_000
-        t0 = time.perf_counter()
-        osrc.intersect_faces(tv, tf[:n_sample], -1)
-        cpu_s = (time.perf_counter() - t0) * (len(tf) / n_sample)
+        cpu_s, ts = median_time(lambda: osrc.intersect_faces(tv, tf, -1), 5)
         out["intersect_faces_cpu_baseline"] = {
-            "value_ms": 1e3 * cpu_s, "cores": int(R.lib().ref_num_threads()), "kind": "port",
-            "sample": f"{n_sample:,} of {len(tf):,} target cells, time scaled by {len(tf) / n_sample:g}",
+            "value_ms": 1e3 * cpu_s, **host, "kind": "port",
+            "sample": f"the whole workload ({len(tf):,} target cells x {len(sf):,} source cells): one warm-up, "
+                      f"median of 5 runs ({min(ts):.3f}-{max(ts):.3f} s)",
         }
         out["intersect_faces_speedup_vs_cpu"] = 1e3 * cpu_s / out["intersect_faces_1Mx250k_ms"]
     return out
@@ -196,6 +279,24 @@ def batch_extras(args):
     trees = [M.CellTree2d(torch.from_numpy(v).to(dev), torch.from_numpy(f).to(dev), -1) for v, f, _, _ in sets]
     torch.cuda.synchronize()
     out["batch_build_trees_ms"] = 1e3 * (time.perf_counter() - t0)
+    # The same 512 trees exported once (.npz in memory) and imported without a rebuild:
+    # what a caller that builds its trees once pays per run instead of the build.
+    import io
+
+    blobs = []
+    for t in trees:
+        buf = io.BytesIO()
+        t.save(buf)
+        blobs.append(buf.getvalue())
+    out["batch_saved_trees_mb"] = sum(len(b) for b in blobs) / 1e6
+    t0 = time.perf_counter()
+    loaded = [M.CellTree2d.load(io.BytesIO(b)) for b in blobs]
+    torch.cuda.synchronize()
+    out["batch_load_trees_ms"] = 1e3 * (time.perf_counter() - t0)
+    out["batch_loaded_trees_identical"] = bool(all(
+        np.array_equal(a.nodes, b.nodes) and np.array_equal(a.bb_indices, b.bb_indices) for a, b in zip(trees, loaded)))
+    trees = loaded  # the timed legs below run on the imported trees
+    del blobs
     d_pts = [torch.from_numpy(p).to(dev) for _, _, p, _ in sets]
     d_seg = [torch.from_numpy(s).to(dev) for _, _, _, s in sets]
     out["batch_meshes"] = len(sets)
@@ -310,11 +411,14 @@ def main() -> int:
     d_f = torch.from_numpy(f).cuda()
     d_pts = torch.from_numpy(pts).cuda()
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    tree = M.CellTree2d(d_v, d_f, -1)
-    torch.cuda.synchronize()
-    build_ms = 1e3 * (time.perf_counter() - t0)
-    log(f"tree: {len(f)} cells, {tree.n_nodes} nodes, depth {tree.depth}, build {build_ms:.1f} ms")
+    build = {}
+    for label in ("build_cold_ms", "build_ms"):  # the first call also loads the code objects
+        t0 = time.perf_counter()
+        tree = M.CellTree2d(d_v, d_f, -1)
+        torch.cuda.synchronize()
+        build[label] = 1e3 * (time.perf_counter() - t0)
+    log(f"tree: {len(f)} cells, {tree.n_nodes} nodes, depth {tree.depth}, build {build['build_ms']:.1f} ms "
+        f"(cold {build['build_cold_ms']:.1f} ms)")
 
     from numba_celltree_amd import _device as D
 
@@ -393,7 +497,8 @@ def main() -> int:
             "parallelism": "single GPU",
         },
         "roofline": roofline,
-        "build_ms": round(build_ms, 2),
+        "build_ms": round(build["build_ms"], 2),
+        "build_cold_ms": round(build["build_cold_ms"], 2),
         "inside_fraction": float((res >= 0).mean()),
     }
     if args.only == "locate":

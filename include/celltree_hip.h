@@ -114,6 +114,18 @@ int ct_build(const double* bb_coords, int64_t n_elements, int32_t n_buckets,
              ct_node* nodes_out, int32_t* bb_indices_out, int64_t* n_nodes, int32_t* depth,
              void* stream);
 
+/* Host-side check of an imported tree (HOST arrays; no device work), the arrays a reference
+ * tree holds as attributes (celltree.py:80-97) in this library's layouts: nodes (n_nodes
+ * ct_node records), bb_indices (n_elements), elements (n_elements, n_max_vert).  Returns
+ * CT_OK and the tree depth in *depth when every child link points forwards inside the node
+ * array, every node but the root has exactly one parent, every node's cell range lies in
+ * [0, n_elements), bb_indices is a permutation and every element index addresses one of
+ * n_vertices vertices (allow_fill: -1 padding past a face's first three entries); else
+ * CT_ERR_ARG with the first violation in ct_last_error().  The traversal kernels rely on
+ * exactly these properties. */
+int ct_check_tree(const ct_node* nodes, int64_t n_nodes, const int32_t* bb_indices, const int32_t* elements,
+                  int64_t n_elements, int32_t n_max_vert, int64_t n_vertices, int32_t allow_fill, int32_t* depth);
+
 /* query.py:568-622 collect_node_bounds -> out (n_nodes, 4); bbox is a HOST array of 4;
  * parent_scratch: device int32 array of n_nodes. */
 int ct_collect_node_bounds(const ct_node* nodes, int64_t n_nodes, const double* bbox,

@@ -6,6 +6,7 @@ device tensor in, device tensor out (no PCIe round trip).
 """
 
 import ctypes
+import warnings
 
 import numpy as np
 import torch
@@ -47,6 +48,17 @@ class Io:
         return t.cpu().numpy()
 
 
+def host_tensor(a: np.ndarray) -> torch.Tensor:
+    """A CPU tensor sharing ``a``'s memory; read-only arrays (views of immutable bytes) are
+    shared too, for copies that only read them."""
+    a = np.ascontiguousarray(a)
+    if a.flags.writeable:
+        return torch.from_numpy(a)
+    with warnings.catch_warnings():
+        warnings.filterwarnings("ignore", message="The given NumPy array is not writable")
+        return torch.from_numpy(a)
+
+
 def to_dev(x, dtype: torch.dtype) -> torch.Tensor:
     """Contiguous, 32-byte aligned device tensor of ``dtype`` (copies host data).
 
@@ -57,7 +69,7 @@ def to_dev(x, dtype: torch.dtype) -> torch.Tensor:
     if isinstance(x, torch.Tensor):
         t = x.to(device=dev, dtype=dtype).contiguous()
     else:
-        t = torch.from_numpy(np.ascontiguousarray(x)).to(device=dev, dtype=dtype).contiguous()
+        t = host_tensor(np.asarray(x)).to(device=dev, dtype=dtype).contiguous()
     if t.data_ptr() % 32:
         t = t.clone()
     return t
@@ -73,7 +85,7 @@ def faces_to_dev(faces, n_vertices: int, allow_fill: bool = True) -> torch.Tenso
     if isinstance(faces, torch.Tensor):
         f = faces
     else:
-        f = torch.from_numpy(np.ascontiguousarray(faces))
+        f = host_tensor(np.asarray(faces))
     if f.numel():
         hi, lo = int(f.max()), int(f.min())
         if hi >= min(int(n_vertices), 2**31):

@@ -157,15 +157,13 @@ class CellTree2dBase:
     KIND = "faces"
 
     def to_arrays(self) -> dict:
-        """The tree as host arrays in the reference's layouts (see numba_celltree_amd.treeio):
-        vertices, elements (CCW faces or edges), nodes (NodeDType), bb_indices, bb_coords,
-        n_buckets, cells_per_leaf and kind.  Copies from the device."""
-        from numba_celltree_amd import _device as _D
-
+        """The tree as host arrays in the reference's layouts (numba_celltree_amd.treeio):
+        vertices, elements (CCW faces or edges, int64), nodes (NodeDType), bb_indices (intp),
+        bb_coords, n_buckets, cells_per_leaf and kind.  Copies from the device."""
         return {
             "vertices": self.d_vertices.cpu().numpy(),
             "elements": self._elements_host().copy(),
-            "nodes": _D.nodes_to_host(self.d_nodes, self.n_nodes),
+            "nodes": D.nodes_to_host(self.d_nodes, self.n_nodes),
             "bb_indices": self.d_bb_indices.cpu().numpy().astype(np.intp),
             "bb_coords": self.d_bb_coords.cpu().numpy(),
             "n_buckets": int(self.n_buckets),
@@ -173,12 +171,28 @@ class CellTree2dBase:
             "kind": self.KIND,
         }
 
+    def _compact_arrays(self) -> dict:
+        """The tree in the compact file form: int32 elements and bb_indices, nodes as device
+        records, no bb_coords (numba_celltree_amd.treeio)."""
+        from numba_celltree_amd import treeio
+
+        return {
+            "vertices": self.d_vertices.cpu().numpy(),
+            "elements": self.d_elements.cpu().numpy(),
+            "nodes": self.d_nodes.cpu().numpy().view(treeio.NODE_RECORD),
+            "bb_indices": self.d_bb_indices.cpu().numpy(),
+            "n_buckets": int(self.n_buckets),
+            "cells_per_leaf": int(self.cells_per_leaf),
+            "kind": self.KIND,
+        }
+
     @classmethod
-    def from_arrays(cls, vertices, elements, nodes, bb_indices, bb_coords, n_buckets: int = 4,
+    def from_arrays(cls, vertices, elements, nodes, bb_indices, bb_coords=None, n_buckets: int = 4,
                     cells_per_leaf: int = 2, kind: Optional[str] = None, **_ignored):
-        """A tree from exported arrays (``to_arrays``, ``load``, or a reference tree's own
-        attributes), without rebuilding it.  The arrays are validated on the host; the device
-        layouts are derived again."""
+        """A tree from exported arrays (``to_arrays``, or a reference tree's own attributes:
+        vertices, faces / edges, nodes, bb_indices, bb_coords), without rebuilding it.  The
+        arrays are validated on the host; given bb_coords must be the elements' bounding
+        boxes; the device layouts are derived again."""
         from numba_celltree_amd import treeio
 
         arrays = treeio.validate({
@@ -186,6 +200,10 @@ class CellTree2dBase:
             "bb_coords": bb_coords, "n_buckets": n_buckets, "cells_per_leaf": cells_per_leaf,
             "kind": cls.KIND if kind is None else kind,
         })
+        return cls._from_validated(arrays)
+
+    @classmethod
+    def _from_validated(cls, arrays: dict):
         if arrays["kind"] != cls.KIND:
             raise ValueError(f"{cls.__name__} holds a {cls.KIND} tree, the arrays are a {arrays['kind']} tree")
         tree = cls.__new__(cls)
@@ -193,43 +211,45 @@ class CellTree2dBase:
         return tree
 
     def _import(self, a: dict) -> None:
+        """Uploads a validated tree (treeio.validate's compact form) and derives the rest."""
         d_vertices = D.to_dev(a["vertices"], torch.float64)
-        allow_fill = a["kind"] == "faces"
-        d_elements = D.faces_to_dev(a["elements"], d_vertices.shape[0], allow_fill=allow_fill)
-        d_bb = D.to_dev(a["bb_coords"], torch.float64)
-        # bb_coords must be the elements' own bounding boxes: queries prune with them.
+        faces = a["kind"] == "faces"
+        d_elements = D.faces_to_dev(a["elements"], d_vertices.shape[0], allow_fill=faces)
         n, nmax = int(d_elements.shape[0]), int(d_elements.shape[1])
-        check = D.empty((n, 4), torch.float64)
         st = D.stream()
-        if allow_fill:
-            _native.call("ct_build_face_bboxes", D.ptr(d_elements), n, nmax, D.ptr(d_vertices), D.ptr(check), st)
+        d_bb = D.empty((n, 4), torch.float64)
+        if faces:
+            _native.call("ct_build_face_bboxes", D.ptr(d_elements), n, nmax, D.ptr(d_vertices), D.ptr(d_bb), st)
         else:
-            _native.call("ct_build_edge_bboxes", D.ptr(d_elements), n, D.ptr(d_vertices), self._edge_tolerance(d_vertices),
-                         D.ptr(check), st)
-        same = (check == d_bb) | (torch.isnan(check) & torch.isnan(d_bb))
-        if not bool(same.all()):
-            raise ValueError("bb_coords are not the bounding boxes of the elements")
-        del check
-        d_nodes = D.nodes_to_device(a["nodes"])
+            _native.call("ct_build_edge_bboxes", D.ptr(d_elements), n, D.ptr(d_vertices),
+                         self._edge_tolerance(d_vertices), D.ptr(d_bb), st)
+        if a.get("bb_coords") is not None:
+            # given boxes must be the elements' own: queries prune with them
+            given = D.to_dev(a["bb_coords"], torch.float64)
+            same = (given == d_bb) | (torch.isnan(given) & torch.isnan(d_bb))
+            if not bool(same.all()):
+                raise ValueError("bb_coords are not the bounding boxes of the elements")
+        d_nodes = D.to_dev(a["nodes"].view(np.uint8), torch.uint8)
         d_bb_indices = D.to_dev(a["bb_indices"], torch.int32)
         self._adopt(d_vertices, d_elements, d_bb, d_nodes, d_bb_indices, len(a["nodes"]), a["depth"],
                     a["n_buckets"], a["cells_per_leaf"])
 
     def save(self, file) -> None:
-        """Writes the tree to ``file`` (a path or file object) as an .npz archive."""
+        """Writes the tree to ``file`` (a path or binary file object) in the compact file form
+        of numba_celltree_amd.treeio."""
         from numba_celltree_amd import treeio
 
-        treeio.save(file, self.to_arrays())
+        treeio.save(file, self._compact_arrays())
 
     @classmethod
     def load(cls, file):
-        """Reads a tree written by ``save`` (validated; not rebuilt)."""
+        """Reads a tree written by ``save`` (validated on the host; not rebuilt)."""
         from numba_celltree_amd import treeio
 
-        return cls.from_arrays(**treeio.load(file))
+        return cls._from_validated(treeio.load(file))
 
     def __getstate__(self):
-        return self.to_arrays()
+        return self._compact_arrays()
 
     def __setstate__(self, state):
         from numba_celltree_amd import treeio

@@ -13,8 +13,11 @@ ct_regrid_vertex_backward).
 """
 
 import ctypes
-from typing import Optional
+import dataclasses
+import weakref
+from typing import Optional, Tuple
 
+import numpy as np
 import torch
 
 from numba_celltree_amd import _device as D
@@ -76,6 +79,36 @@ class RegridTopology:
         s.n_tgt_vertices, s.n_src_vertices = self.n_tgt_vertices, self.n_src_vertices
         s.n_max_t, s.n_max_s = int(d_tf.shape[1]), int(self.src_faces.shape[1])
         self._struct = s
+        self._area_cache = None
+
+    # ------------------------------------------------------------- area cache
+    def areas(self, src_v: torch.Tensor, tgt_v: torch.Tensor, cache: bool = True) -> Tuple[torch.Tensor, torch.Tensor]:
+        """(pair areas, target areas) for these vertex tensors.  With ``cache``, the last
+        result is kept and reused while the same two tensor objects are passed unchanged
+        (torch's in-place version counters unchanged), so repeated forwards with fixed
+        vertices only apply the operator.  Writes that bypass the version counter (through
+        ``tensor.data``) are not seen: ``clear_cache`` drops the result.  The regrid forward
+        caches only when no vertex gradient is requested (gradcheck, for one, perturbs its
+        inputs through ``.data``)."""
+        c = self._area_cache
+        key = (src_v._version, tgt_v._version)
+        if cache and c is not None and c[0]() is src_v and c[1]() is tgt_v and c[2] == key:
+            return c[3], c[4]
+        area = self.pair_areas(src_v, tgt_v)
+        tarea = self.target_areas(tgt_v)
+        if cache:
+            self._area_cache = (weakref.ref(src_v), weakref.ref(tgt_v), key, area, tarea)
+        return area, tarea
+
+    def clear_cache(self) -> None:
+        self._area_cache = None
+
+    def overlap_sums(self, area: torch.Tensor) -> torch.Tensor:
+        """S_t = sum of the overlap areas of target t's pairs (0 for a target without pairs),
+        summed in pair order (deterministic)."""
+        ones_s = torch.ones((self.n_src, 1), dtype=torch.float64, device=area.device)
+        ones_t = torch.ones(self.n_tgt, dtype=torch.float64, device=area.device)
+        return self.apply(area, ones_t, ones_s)[:, 0].contiguous()
 
     # ------------------------------------------------------------------ kernels
     def pair_areas(self, src_v: torch.Tensor, tgt_v: torch.Tensor) -> torch.Tensor:
@@ -134,38 +167,92 @@ def _require(name: str, t, shape) -> None:
         raise ValueError(f"{name} must have shape {tuple(shape)}, got {tuple(t.shape)}")
 
 
+NORMALIZATIONS = ("target", "overlap", "none")
+
+
 class RegridFunction(torch.autograd.Function):
     """y = regrid(F, src_vertices, tgt_vertices) under a fixed RegridTopology.
 
     F (n_src, n_fields), src_vertices (n_src_vertices, 2) and tgt_vertices
     (n_tgt_vertices, 2) must be float64 tensors on the current GPU (``Regridder`` converts
-    other inputs)."""
+    other inputs).  ``normalize``: "target" divides each target's sum by its polygon area
+    (conservative regridding; differentiable in F and both vertex sets), "overlap" by the sum
+    of its overlap areas (the area-weighted mean of examples/spatial_indexing_2d_grids.py:
+    285-298; NaN for a target that overlaps no source; differentiable in F only), "none"
+    leaves the plain sum of area-weighted values."""
 
     @staticmethod
-    def forward(ctx, F, src_vertices, tgt_vertices, topo: RegridTopology):
+    def forward(ctx, F, src_vertices, tgt_vertices, topo: RegridTopology, normalize: str = "target"):
         _require("F", F, (topo.n_src, None))
         _require("src_vertices", src_vertices, (topo.n_src_vertices, 2))
         _require("tgt_vertices", tgt_vertices, (topo.n_tgt_vertices, 2))
+        if normalize not in NORMALIZATIONS:
+            raise ValueError(f"normalize must be one of {NORMALIZATIONS}")
+        vertex_grad = ctx.needs_input_grad[1] or ctx.needs_input_grad[2]
+        area, tarea = topo.areas(src_vertices, tgt_vertices, cache=not vertex_grad)
+        if normalize == "overlap":
+            div = topo.overlap_sums(area)
+        elif normalize == "none":
+            div = torch.ones_like(tarea)
+        else:
+            div = tarea
         F = F.contiguous()
-        src_vertices = src_vertices.contiguous()
-        tgt_vertices = tgt_vertices.contiguous()
-        area = topo.pair_areas(src_vertices, tgt_vertices)
-        tarea = topo.target_areas(tgt_vertices)
-        y = topo.apply(area, tarea, F)
+        y = topo.apply(area, div, F)
         ctx.topo = topo
-        ctx.save_for_backward(F, src_vertices, tgt_vertices, area, tarea, y)
+        ctx.normalize = normalize
+        ctx.save_for_backward(F, src_vertices.contiguous(), tgt_vertices.contiguous(), area, div, y)
         return y
 
     @staticmethod
     def backward(ctx, gy):
-        F, src_v, tgt_v, area, tarea, y = ctx.saved_tensors
+        F, src_v, tgt_v, area, div, y = ctx.saved_tensors
         topo = ctx.topo
         gy = gy.to(dtype=torch.float64).contiguous()
-        gF = topo.apply_transpose(area, tarea, gy) if ctx.needs_input_grad[0] else None
+        gF = topo.apply_transpose(area, div, gy) if ctx.needs_input_grad[0] else None
         g_sv = g_tv = None
         if ctx.needs_input_grad[1] or ctx.needs_input_grad[2]:
-            g_sv, g_tv = topo.vertex_backward(src_v, tgt_v, F, y, gy, tarea)
-        return gF, g_sv, g_tv, None
+            if ctx.normalize != "target":
+                raise NotImplementedError("vertex gradients are implemented for normalize='target'")
+            g_sv, g_tv = topo.vertex_backward(src_v, tgt_v, F, y, gy, div)
+        return gF, g_sv, g_tv, None, None
+
+
+@dataclasses.dataclass
+class RegridWeights:
+    """The regridding operator as a sparse (n_tgt, n_src) matrix in CSR form, rows grouped by
+    target in intersect_faces order: y = W @ F.  ``values`` are the overlap areas divided by
+    the normaliser (see RegridFunction); ``coo()`` gives the (target, source, value) lists the
+    reference's intersect_* methods return (examples/spatial_indexing_2d_grids.py:268-323)."""
+
+    crow_indices: torch.Tensor  # (n_tgt + 1,) int64
+    col_indices: torch.Tensor   # (n_pairs,) int64 source cells
+    values: torch.Tensor        # (n_pairs,) float64
+    shape: Tuple[int, int]
+    _topo: "RegridTopology" = dataclasses.field(repr=False, default=None)
+
+    def coo(self):
+        rows = torch.repeat_interleave(
+            torch.arange(self.shape[0], device=self.values.device), torch.diff(self.crow_indices))
+        return rows, self.col_indices, self.values
+
+    def to_torch(self) -> torch.Tensor:
+        return torch.sparse_csr_tensor(self.crow_indices, self.col_indices, self.values, size=self.shape)
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+
+        return sp.csr_matrix((self.values.cpu().numpy(), self.col_indices.cpu().numpy(),
+                              self.crow_indices.cpu().numpy()), shape=self.shape)
+
+    def apply(self, F) -> torch.Tensor:
+        """W @ F for F (n_src,) or (n_src, n_fields): a deterministic segmented sum per row."""
+        F = _as_f64(F)
+        squeeze = F.ndim == 1
+        F2 = (F[:, None] if squeeze else F).contiguous()
+        _require("F", F2, (self.shape[1], None))
+        ones = torch.ones(self.shape[0], dtype=torch.float64, device=F2.device)
+        y = self._topo.apply(self.values, ones, F2)
+        return y[:, 0] if squeeze else y
 
 
 def _as_f64(x) -> torch.Tensor:
@@ -182,8 +269,29 @@ class Regridder:
     >>> y = rg(F, src_vertices=v_s, tgt_vertices=v_t)   # differentiable in all three
     """
 
-    def __init__(self, tree, tgt_vertices, tgt_faces, fill_value: int = -1):
+    def __init__(self, tree, tgt_vertices, tgt_faces, fill_value: int = -1, normalize: str = "target"):
+        if normalize not in NORMALIZATIONS:
+            raise ValueError(f"normalize must be one of {NORMALIZATIONS}")
         self.topology = RegridTopology(tree, tgt_vertices, tgt_faces, fill_value)
+        self.normalize = normalize
+
+    def weights(self, src_vertices: Optional[torch.Tensor] = None,
+                tgt_vertices: Optional[torch.Tensor] = None) -> RegridWeights:
+        """The operator of this regridder (at the given vertices) as a CSR matrix."""
+        topo = self.topology
+        src_vertices = topo.tree.d_vertices if src_vertices is None else _as_f64(src_vertices)
+        tgt_vertices = topo.tgt_vertices if tgt_vertices is None else _as_f64(tgt_vertices)
+        _require("src_vertices", src_vertices, (topo.n_src_vertices, 2))
+        _require("tgt_vertices", tgt_vertices, (topo.n_tgt_vertices, 2))
+        with torch.no_grad():
+            area, tarea = topo.areas(src_vertices, tgt_vertices)
+            if self.normalize == "target":
+                values = area / tarea[topo.tt]
+            elif self.normalize == "overlap":
+                values = area / topo.overlap_sums(area)[topo.tt]
+            else:
+                values = area.clone()
+        return RegridWeights(topo.tgt_offsets, topo.ss, values, (topo.n_tgt, topo.n_src), topo)
 
     def __call__(self, F, src_vertices: Optional[torch.Tensor] = None,
                  tgt_vertices: Optional[torch.Tensor] = None) -> torch.Tensor:
@@ -198,5 +306,5 @@ class Regridder:
         squeeze = F.ndim == 1
         if squeeze:
             F = F[:, None]
-        y = RegridFunction.apply(F, src_vertices, tgt_vertices, topo)
+        y = RegridFunction.apply(F, src_vertices, tgt_vertices, topo, self.normalize)
         return y[:, 0] if squeeze else y

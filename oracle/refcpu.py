@@ -79,9 +79,55 @@ class _Pairs(ctypes.Structure):
 
 _lib = None
 
+# CPU features each ISA level of build_oracle.py needs (the x86-64 psABI levels).
+ISA_FEATURES = {
+    "x86-64-v4": ("avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"),
+    "x86-64-v3": ("avx2", "bmi1", "bmi2", "fma", "movbe", "abm", "f16c"),
+}
+
+
+def host_cpu():
+    """(model name, flags) of this host's CPU from /proc/cpuinfo ("unknown", set() elsewhere)."""
+    model, flags = "unknown", set()
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name") and model == "unknown":
+                    model = line.split(":", 1)[1].strip()
+                elif line.startswith("flags") and not flags:
+                    flags = set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return model, flags
+
+
+def native_isa_level() -> str:
+    """The highest ISA level of build_oracle.py this host's CPU executes."""
+    _, flags = host_cpu()
+    for level in ("x86-64-v4", "x86-64-v3"):
+        if set(ISA_FEATURES[level]) <= flags:
+            return level
+    return "x86-64-v2"
+
+
+def use_isa_level(level: str) -> str:
+    """Load the oracle compiled for ``level`` from now on (same results at every level);
+    returns the library path."""
+    global _lib, _LIB_PATH
+    from oracle import build_oracle
+
+    path = build_oracle.library_path(level)
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is not built (python oracle/build_oracle.py)")
+    if path != _LIB_PATH:
+        _LIB_PATH = path
+        _lib = None
+    return path
+
 
 def lib():
-    """Load ``librefcpu.so``, building it first when it is absent."""
+    """Load the oracle library (``librefcpu.so`` unless use_isa_level chose another),
+    building it first when it is absent."""
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):

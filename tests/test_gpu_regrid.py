@@ -330,3 +330,100 @@ def test_inputs_are_checked_or_converted():
         barycentric_weights(tri, pts, vertices=tri.d_vertices[:-1])
     with pytest.raises(TypeError):
         BarycentricTriangleFunction.apply(pts.float(), tri.d_vertices, tri.d_elements, face, 1e-12)
+
+
+# ------------------------------------------------------------- operator assembly
+def test_weights_match_oracle_and_dense_example():
+    """Regridder.weights(): the CSR operator's (row, col) lists equal the oracle's
+    intersect_faces pairs, its values are the oracle's areas over the target area (or over
+    the overlap sum), and applying it (W.apply, torch sparse, scipy) equals the dense
+    weight-matrix recipe of the reference's example (examples/spatial_indexing_2d_grids.py:
+    268-298): np.dot(weight_matrix, z) / area_sum, NaN where nothing overlaps."""
+    import torch
+
+    from numba_celltree_amd.regrid import Regridder
+
+    sv, sf, tv, tf, F = problem(41, 20, 10.0, n_fields=3)
+    tv = tv * 1.15 - 0.5  # targets past the source mesh: partly covered and uncovered cells
+    tree, rg = build(sv, sf, tv, tf)
+    o = R.OracleCellTree2d(sv, sf, -1)
+    i, j, area = o.intersect_faces(tv, tf, -1)
+    ccw = tf.astype(np.int64).copy()
+    R.counter_clockwise(tv, ccw)
+    tarea = np.array([R.polygon_area(tv[row[row >= 0]]) for row in ccw])
+
+    W = rg.weights()
+    rows, cols, vals = (x.cpu().numpy() for x in W.coo())
+    assert np.array_equal(rows, i) and np.array_equal(cols, j)
+    np.testing.assert_allclose(vals, area / tarea[i], rtol=1e-12)
+    assert W.shape == (len(tf), len(sf))
+
+    dense = np.zeros((len(tf), len(sf)))
+    dense[i, j] = area
+    z = F[:, 0]
+    area_sum = dense @ np.ones(len(sf))
+    want = np.full(len(tf), np.nan)
+    hit = area_sum > 0
+    want[hit] = (dense @ z)[hit] / area_sum[hit]
+    assert (~hit).any() and (area_sum[hit] < tarea[hit] * (1 - 1e-6)).any()
+
+    mean = Regridder(tree, tv, tf, -1, normalize="overlap")
+    got = mean(torch.from_numpy(z).cuda()).cpu().numpy()
+    assert np.array_equal(np.isnan(got), np.isnan(want))
+    np.testing.assert_allclose(got[hit], want[hit], rtol=1e-12)
+    Wm = mean.weights()
+    np.testing.assert_allclose(Wm.apply(z).cpu().numpy()[hit], want[hit], rtol=1e-12)
+    np.testing.assert_allclose(Wm.to_scipy() @ z, np.nan_to_num(want, nan=0.0), rtol=1e-12, atol=1e-300)
+    # conservative form: divided by the target's own area
+    y = rg(torch.from_numpy(F).cuda()).cpu().numpy()
+    np.testing.assert_allclose(y, (dense @ F) / tarea[:, None], rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(W.apply(F).cpu().numpy(), y, rtol=1e-12, atol=1e-14)
+    ts = (W.to_torch() @ torch.from_numpy(F).cuda()).cpu().numpy()
+    np.testing.assert_allclose(ts, y, rtol=1e-12, atol=1e-14)
+    raw = Regridder(tree, tv, tf, -1, normalize="none")
+    np.testing.assert_allclose(raw(torch.from_numpy(F).cuda()).cpu().numpy(), dense @ F, rtol=1e-12, atol=1e-14)
+
+
+def test_areas_cached_while_vertices_unchanged(monkeypatch):
+    """With the same, unmodified vertex tensors the forward reuses the pair areas (the clip
+    kernel does not run again); an in-place change of either vertex array recomputes them,
+    and the result equals a regridder built on the changed mesh."""
+    import torch
+
+    from numba_celltree_amd import regrid as RG
+
+    sv, sf, tv, tf, F = problem(33, 12, 4.0, n_fields=2, inset=0.0731)
+    tree, rg = build(sv, sf, tv, tf)
+    topo = rg.topology
+    calls = []
+    real = RG.RegridTopology.pair_areas
+
+    def counted(self, a, b):
+        calls.append(1)
+        return real(self, a, b)
+
+    monkeypatch.setattr(RG.RegridTopology, "pair_areas", counted)
+    Fd = torch.from_numpy(F).cuda()
+    svd = tree.d_vertices.clone()
+    tvd = topo.tgt_vertices.clone()
+    y0 = rg(Fd, svd, tvd)
+    y1 = rg(Fd, svd, tvd)
+    rg(Fd)
+    rg(Fd)
+    assert len(calls) == 2 and torch.equal(y0, y1)
+    with torch.no_grad():
+        svd[5] += 0.01  # in place: version counter moves
+    y2 = rg(Fd, svd, tvd)
+    assert len(calls) == 3 and not torch.equal(y2, y0)
+    fresh = RG.Regridder(tree, tv, tf, -1).topology
+    a = real(fresh, svd, tvd)  # not counted
+    assert torch.equal(fresh.apply(a, fresh.target_areas(tvd), Fd), y2)
+    topo.clear_cache()
+    rg(Fd, svd, tvd)
+    assert len(calls) == 4
+    # with a vertex gradient requested the areas are always recomputed (gradcheck, for one,
+    # perturbs its inputs through .data, which the version counter does not see)
+    svg = svd.clone().requires_grad_(True)
+    rg(Fd, svg, tvd)
+    rg(Fd, svg, tvd)
+    assert len(calls) == 6

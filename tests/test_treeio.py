@@ -37,7 +37,8 @@ def oracle_tree():
 def test_validate_accepts_reference_layout_and_finds_depth(oracle_tree):
     a = treeio.validate(_arrays(oracle_tree))
     assert a["depth"] == _depth_by_recursion(oracle_tree.nodes)
-    assert np.array_equal(a["nodes"], oracle_tree.nodes)
+    assert np.array_equal(treeio.to_node_dtype(a["nodes"]), oracle_tree.nodes)
+    assert a["nodes"].dtype.itemsize == 32 and a["elements"].dtype == np.int32
     v, f = S.jittered_grid(61, 100.0, 0.25, 1.0 / 3.0, 2)
     mixed = R.OracleCellTree2d(v, f, -1, n_buckets=3, cells_per_leaf=1)
     assert treeio.validate(_arrays(mixed))["depth"] == _depth_by_recursion(mixed.nodes)
@@ -51,9 +52,46 @@ def test_save_load_round_trip(oracle_tree):
     treeio.save(buf, _arrays(oracle_tree))
     buf.seek(0)
     a = treeio.load(buf)
-    for k in ("vertices", "elements", "nodes", "bb_indices", "bb_coords"):
+    for k in ("vertices", "elements", "bb_indices"):
         assert np.array_equal(a[k], getattr(oracle_tree, "faces" if k == "elements" else k)), k
+    assert np.array_equal(treeio.to_node_dtype(a["nodes"]), oracle_tree.nodes)
+    assert a["bb_coords"] is None  # not stored: recomputed on import
     assert (a["n_buckets"], a["cells_per_leaf"], a["kind"]) == (4, 2, "faces")
+    assert a["depth"] == _depth_by_recursion(oracle_tree.nodes)
+
+
+def test_file_corruption_is_rejected(oracle_tree):
+    """Truncations, a bad magic, a bad header and out-of-range array entries raise
+    ValueError; a flipped child link inside the node array fails validation."""
+    import json
+
+    buf = io.BytesIO()
+    treeio.save(buf, _arrays(oracle_tree))
+    raw = buf.getvalue()
+    for bad in (raw[:10], b"NOTATREE" + raw[8:], raw[: len(raw) // 2]):
+        with pytest.raises(ValueError):
+            treeio.load(bad)
+    hlen = int(np.frombuffer(raw, dtype="<u8", count=1, offset=8)[0])
+    header = json.loads(raw[16:16 + hlen])
+    for mutate in (lambda h: h["arrays"][2].__setitem__("dtype", "|O"),
+                   lambda h: h["arrays"][0].__setitem__("shape", [10**9, 2]),
+                   lambda h: h["arrays"][1].__setitem__("offset", -64),
+                   lambda h: h.__setitem__("format", "x")):
+        h = json.loads(json.dumps(header))
+        mutate(h)
+        hb = json.dumps(h).encode()
+        hb = hb + b" " * (hlen - len(hb)) if len(hb) <= hlen else None
+        if hb is None:
+            continue
+        with pytest.raises(ValueError):
+            treeio.load(raw[:16] + hb + raw[16 + hlen:])
+    # a child link pointing backwards inside the node records
+    nodes_at = 16 + hlen + (-(16 + hlen)) % 64 + header["arrays"][2]["offset"]
+    rec = bytearray(raw)
+    internal = int(np.flatnonzero(oracle_tree.nodes["child"] != -1)[1])
+    rec[nodes_at + 32 * internal + 16: nodes_at + 32 * internal + 20] = np.int32(0).tobytes()
+    with pytest.raises(ValueError, match="invalid tree"):
+        treeio.load(bytes(rec))
 
 
 def _corrupt(base, key, fn):
