@@ -28,11 +28,12 @@ extern "C" {
 #define CT_ERR_ARG (-1)        /* invalid argument (shape, size, null pointer) */
 #define CT_ERR_HIP (-2)        /* a HIP runtime call failed */
 #define CT_ERR_DEGENERATE (-3) /* tree build: a centroid falls in no bucket (the reference raises IndexError) */
-#define CT_ERR_DEPTH (-4)      /* tree deeper than the largest traversal stack compiled in */
+#define CT_ERR_DEPTH (-4)      /* tree deeper than CT_MAX_DEPTH without ct_tree.parents */
 #define CT_ERR_WORKSPACE (-5)  /* workspace smaller than the matching *_workspace_bytes() */
 
 #define CT_MAX_N_VERTEX 32     /* constants.py:134 */
-#define CT_MAX_DEPTH 254       /* deepest tree the traversal kernels accept */
+#define CT_MAX_DEPTH 254       /* deepest tree the stack traversals take; deeper trees are walked
+                                  through parent links (ct_tree.parents, see ct_node_parents) */
 
 /* One tree node, 32 bytes, the device form of NodeDType (constants.py:92-106).
  * child == -1 marks a leaf; the right child is child + 1. */
@@ -69,6 +70,9 @@ typedef struct ct_tree {
   int32_t reserved;          /* 0 */
   const double* leaf_bb;     /* optional (may be NULL): (n_elements, 4), row k = the bbox of
                                 cell bb_indices[k]; see ct_pack_leaf_bboxes */
+  const int32_t* parents;    /* (n_nodes) parent of every node (-1 for the root), required when
+                                depth > CT_MAX_DEPTH, else unused (may be NULL); see
+                                ct_node_parents */
 } ct_tree;
 
 const char* ct_last_error(void);
@@ -148,6 +152,13 @@ int ct_locate_points(const ct_tree* tree, const double* points, int64_t n, doubl
  * a fixed n_max_vert-edge loop (a repeated vertex adds a zero-length edge, which changes no
  * point-in-polygon answer).  Requires 3 <= n_max_vert <= 8. */
 int ct_pack_leaf_coords(const ct_tree* tree, double* out, void* stream);
+
+/* Layout helper, no reference counterpart: out (n_nodes) gets the parent of every node (-1
+ * for the root).  Trees deeper than CT_MAX_DEPTH need it (ct_tree.parents): their queries
+ * keep no traversal stack (the reference's grows without bound, utils.py:34-97) and find a
+ * deferred sibling again by climbing these links.  Results are the same as the stack
+ * traversal's. */
+int ct_node_parents(const ct_tree* tree, int32_t* out, void* stream);
 
 /* Layout helper, no reference counterpart: out (n_elements, 4) gets, at row k, the bbox
  * (xmin, xmax, ymin, ymax) of cell bb_indices[k], so box and segment queries of a tree with

@@ -56,6 +56,7 @@ class CtTree(ctypes.Structure):
         ("start_grid_n", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
         ("leaf_bb", ctypes.c_void_p),
+        ("parents", ctypes.c_void_p),
     ]
 
 
@@ -113,7 +114,7 @@ class CtRegridTopology(ctypes.Structure):
 
 
 assert ctypes.sizeof(CtNode) == 32
-assert ctypes.sizeof(CtTree) == 144
+assert ctypes.sizeof(CtTree) == 152
 assert ctypes.sizeof(CtBatchTile) == 16
 
 _P = ctypes.c_void_p
@@ -151,6 +152,7 @@ SIGNATURES = {
     "ct_locate_points_ordered": (ctypes.c_int, [_TP, _P, _I64, _F64, _P, _SZ, _P, _P]),
     "ct_locate_points_in_order": (ctypes.c_int, [_TP, _P, _I64, _P, _P, _F64, _P, _P]),
     "ct_check_tree": (ctypes.c_int, [_P, _I64, _P, _P, _I64, _I32, _I64, _I32, _P]),
+    "ct_node_parents": (ctypes.c_int, [_TP, _P, _P]),
     "ct_locate_points_ordered_noalloc": (ctypes.c_int, [_TP, _P, _I64, _F64, _P, _P]),
     "ct_locate_points_on_edge": (ctypes.c_int, [_TP, _P, _I64, _F64, _P, _P]),
     "ct_order_workspace_bytes": (_SZ, [_I64]),

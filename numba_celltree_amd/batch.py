@@ -72,6 +72,9 @@ class TreeBatch:
             nm = int(t.d_elements.shape[1])
             if not hasattr(t, "d_node_boxes") or not 3 <= nm <= 8:
                 raise ValueError("batched queries take CellTree2d trees with 3..8 vertices per face")
+            if int(t.depth) > _native.CT_MAX_DEPTH:
+                raise ValueError(f"batched queries take trees of depth <= {_native.CT_MAX_DEPTH} (this one: "
+                                 f"{int(t.depth)}); query deeper trees one by one")
         self.trees = trees
         self.n_trees = len(trees)
         raw = b"".join(ctypes.string_at(ctypes.addressof(t._ctree), ctypes.sizeof(t._ctree)) for t in trees)

@@ -133,6 +133,13 @@ class CellTree2dBase:
             self.d_leaf_bb = D.empty((n, 4), torch.float64)
             _native.call("ct_pack_leaf_bboxes", self.tree_ref, D.ptr(self.d_leaf_bb), st)
             t.leaf_bb = self.d_leaf_bb.data_ptr()
+        t.parents = None
+        self.d_parents = None
+        if self.depth > _native.CT_MAX_DEPTH:
+            # Deeper than the largest traversal stack: queries climb parent links instead.
+            self.d_parents = D.empty(self.n_nodes, torch.int32)
+            _native.call("ct_node_parents", self.tree_ref, D.ptr(self.d_parents), st)
+            t.parents = self.d_parents.data_ptr()
         t.start_grid = None
         t.start_grid_n = 0
         self.d_start_grid = None

@@ -6,6 +6,7 @@
 // (`order`, from ct_query_order): every query writes only its own count / rows.
 #include <mutex>
 
+#include "ct_deep.h"
 #include "ct_internal.h"
 #include "ct_primitives.h"
 #include "ct_traverse.h"
@@ -731,6 +732,10 @@ __global__ void __launch_bounds__(kBlock) k_node_boxes_round(TreeView t, int64_t
   stamp[x] = r;
 }
 
+// Longest parent chain k_node_bounds follows: deeper than any tree it is given (a chain that
+// long means edited child links formed a cycle; the climb then stops).
+constexpr int kClimbLimit = 1 << 16;
+
 // query.py:568-622 collect_node_bounds: a child copies its parent's bounds and replaces
 // slot dim*2 + side (left: Lmax into the max slot, right: Rmin into the min slot).  Walking
 // up from a node, the first ancestor edge that writes a slot decides it; the root bbox
@@ -743,7 +748,7 @@ __global__ void k_node_bounds(const ct_node* __restrict__ nodes, int64_t n_nodes
   double v[4] = {b0, b1, b2, b3};
   bool set[4] = {false, false, false, false};
   int64_t y = x;
-  for (int guard = 0; guard < 4096 && y > 0; ++guard) {
+  for (int guard = 0; guard < kClimbLimit && y > 0; ++guard) {
     const int p = parent[y];
     if (p < 0 || p >= n_nodes) break;
     const ct_node pn = nodes[p];
@@ -784,15 +789,16 @@ __global__ void k_validate(TreeView t, int64_t n_nodes, const double* __restrict
 }
 
 // ----------------------------------------------------------------- dispatch
+// Trees deeper than CT_MAX_DEPTH are walked without a stack (ct_deep.hip); every traversal
+// launcher below hands them over first.
 int check_tree(const ct_tree* tree) {
   CT_REQUIRE(tree != nullptr && tree->nodes && tree->bb_indices && tree->elements && tree->vertices,
              "incomplete tree");
-  if (tree->depth > CT_MAX_DEPTH) {
-    set_error("tree depth %d exceeds the supported %d", tree->depth, CT_MAX_DEPTH);
-    return CT_ERR_DEPTH;
-  }
+  CT_REQUIRE(tree->depth >= 0, "negative tree depth");
   return CT_OK;
 }
+
+bool is_deep(const ct_tree* tree) { return tree->depth > CT_MAX_DEPTH; }
 
 template <int MAXV, bool ON_EDGE, int LEAF>
 struct LocateLaunch {
@@ -820,6 +826,7 @@ int launch_locate_v(const TreeView& t, int depth, const double* pts, QueryOrder 
 // Face trees: picks the vertex-count class, and the packed leaf path when present.
 int launch_locate_faces(const ct_tree* tree, const double* pts, QueryOrder order, int64_t n,
                         double tol, int64_t* out, hipStream_t st) {
+  if (is_deep(tree)) return deep_locate_points(tree, pts, order.order, n, tol, false, out, st);
   const TreeView t = view(tree);
   const int nm = tree->n_max_vert, d = tree->depth;
   if (t.leaf != nullptr && nm <= 8) {
@@ -882,6 +889,8 @@ template <bool WRITE>
 int launch_edges(const ct_tree* tree, const double* segs, QueryOrder order, int64_t n, int kind,
                  int64_t* counts, const int64_t* offsets, int64_t* ii, int64_t* jj, double* xy,
                  int32_t* hits, int slots, hipStream_t st) {
+  if (is_deep(tree))  // no hit record: both passes walk
+    return deep_edges(tree, segs, order.order, n, kind == 1, WRITE, counts, offsets, ii, jj, xy, st);
   const TreeView t = view(tree);
   const int d = tree->depth, nm = tree->n_max_vert;
   if (kind == 0)
@@ -1098,6 +1107,7 @@ int ct_locate_points_on_edge(const ct_tree* tree, const double* points, int64_t 
   CT_REQUIRE(n >= 0, "n must be >= 0");
   CT_REQUIRE(tree->n_max_vert == 2, "edge tree expected (n_max_vert == 2)");
   if (n == 0) return CT_OK;
+  if (is_deep(tree)) return deep_locate_points(tree, points, nullptr, n, tol, true, out, as_stream(stream));
   return launch_locate_v<4, true, kLeafFaces>(view(tree), tree->depth, points, QueryOrder{nullptr, nullptr}, n, tol,
                                          out, as_stream(stream));
 }
@@ -1111,6 +1121,8 @@ int ct_locate_boxes_count_rec(const ct_tree* tree, const double* boxes, int64_t 
   CT_REQUIRE(sorted == nullptr || order != nullptr, "sorted queries need their order");
   CT_REQUIRE(hits == nullptr || (hit_slots >= 1 && hit_slots <= 1024), "hit_slots must be in [1, 1024]");
   if (n == 0) return CT_OK;
+  if (is_deep(tree))  // no hit record: both passes walk
+    return deep_boxes(tree, boxes, order, n, false, counts, nullptr, nullptr, nullptr, as_stream(stream));
   const TreeView t = view(tree);
   return with_stack(tree->depth, BoxesLaunch<false>{t, boxes, QueryOrder{order, sorted}, n, counts, nullptr,
                                                     nullptr, nullptr, hits, hits ? hit_slots : 0,
@@ -1126,6 +1138,7 @@ int ct_locate_boxes_write_rec(const ct_tree* tree, const double* boxes, int64_t 
   CT_REQUIRE(sorted == nullptr || order != nullptr, "sorted queries need their order");
   CT_REQUIRE(hits == nullptr || (hit_slots >= 1 && hit_slots <= 1024), "hit_slots must be in [1, 1024]");
   if (n == 0) return CT_OK;
+  if (is_deep(tree)) return deep_boxes(tree, boxes, order, n, true, nullptr, offsets, ii, jj, as_stream(stream));
   const TreeView t = view(tree);
   return with_stack(tree->depth, BoxesLaunch<true>{t, boxes, QueryOrder{order, sorted}, n, nullptr, offsets, ii,
                                                    jj, const_cast<int32_t*>(hits), hits ? hit_slots : 0,
@@ -1216,9 +1229,14 @@ int ct_compute_barycentric_weights(const ct_tree* tree, const double* points, in
   if (rc) return rc;
   CT_REQUIRE(n >= 0 && tree->n_max_vert >= 3, "bad arguments");
   if (n == 0) return CT_OK;
-  const TreeView t = view(tree);
   hipStream_t st = as_stream(stream);
   const int nm = tree->n_max_vert, d = tree->depth;
+  if (is_deep(tree)) {  // locate, then the weights of the located faces (same arithmetic)
+    rc = deep_locate_points(tree, points, nullptr, n, tol, false, face_out, st);
+    if (rc) return rc;
+    return launch_bary(points, face_out, n, tree->elements, nm, tree->vertices, tol, nm > 3, weights, st);
+  }
+  const TreeView t = view(tree);
   if (t.leaf != nullptr && nm <= 8) {
     if (nm <= 3) return with_stack(d, LocateBaryLaunch<3, kLeafPackedExact>{t, points, n, tol, face_out, weights, st});
     if (nm <= 4) return with_stack(d, LocateBaryLaunch<4, kLeafPackedExact>{t, points, n, tol, face_out, weights, st});
@@ -1241,6 +1259,18 @@ int ct_collect_node_bounds(const ct_node* nodes, int64_t n_nodes, const double* 
   CT_LAUNCH_CHECK();
   k_node_bounds<<<grid_for(n_nodes), kBlock, 0, st>>>(nodes, n_nodes, parent_scratch, bbox[0], bbox[1],
                                                       bbox[2], bbox[3], out);
+  CT_LAUNCH_CHECK();
+  return CT_OK;
+}
+
+int ct_node_parents(const ct_tree* tree, int32_t* out, void* stream) {
+  int rc = check_tree(tree);
+  if (rc) return rc;
+  CT_REQUIRE(tree->n_nodes >= 1 && out, "bad arguments");
+  hipStream_t st = as_stream(stream);
+  rc = fill<int32_t>(out, tree->n_nodes, -1, st);
+  if (rc) return rc;
+  k_parents<<<grid_for(tree->n_nodes), kBlock, 0, st>>>(tree->nodes, tree->n_nodes, out);
   CT_LAUNCH_CHECK();
   return CT_OK;
 }

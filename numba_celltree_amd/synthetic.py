@@ -224,3 +224,28 @@ def config5(n_meshes: int = 512, seed: int = 0, cells=(1e3, 2e5), n_points=(1e3,
         o = origin[m]
         out.append((v + o, f, p + o, s + o[None, None, :]))
     return out
+
+
+def graded_corner(levels: int, ratio: float = 0.5):
+    """A unit square refined geometrically toward the origin: at each of `levels` levels the
+    current square [0, s]^2 is cut into four quadrants of side h = ratio * s ... (ratio 0.5:
+    the usual quadtree corner refinement); the three quadrants away from the origin become two
+    triangles each (sharing their bounding box), and the refinement continues in the quadrant
+    at the origin, whose last square becomes two triangles too.  6 * levels + 2 triangles; the
+    cell tree of this mesh is about `levels` deep, with overlapping child boxes at every level."""
+    verts, faces = [], []
+
+    def v(x, y):
+        verts.append((x, y))
+        return len(verts) - 1
+
+    s = 1.0
+    for _ in range(levels):
+        h = s * ratio
+        for x0, y0, w, hh in ((h, 0.0, s - h, h), (0.0, h, h, s - h), (h, h, s - h, s - h)):
+            a, b, c, d = v(x0, y0), v(x0 + w, y0), v(x0 + w, y0 + hh), v(x0, y0 + hh)
+            faces += [(a, b, c), (a, c, d)]
+        s = h
+    a, b, c, d = v(0.0, 0.0), v(s, 0.0), v(s, s), v(0.0, s)
+    faces += [(a, b, c), (a, c, d)]
+    return np.array(verts, dtype=np.float64), np.array(faces, dtype=np.int32)
