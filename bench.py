@@ -212,6 +212,32 @@ def extras(args):
             "tests/golden/flops_overlap.json regrid.clip_flops_kept: Sutherland-Hodgman + polygon_area ops "
             "of every kept pair (oracle-counted)")
     del tree, rg, topo, W
+    # compute_barycentric_weights with its adjoint: 1e6 points on the 20,000-triangle mesh
+    # (points uniform over [-0.5, 10.5]^2; faces located once, then weights forward / backward).
+    from numba_celltree_amd.barycentric import barycentric_weights
+
+    v1, f1, _ = S.config1()
+    tri = M.CellTree2d(torch.from_numpy(v1).cuda(), torch.from_numpy(f1).cuda(), -1)
+    bp = torch.from_numpy(np.random.default_rng(4).uniform(-0.5, 10.5, size=(1_000_000, 2))).cuda()
+    Gb = torch.randn(1_000_000, 3, dtype=torch.float64, device="cuda")
+
+    def bary_fwd():
+        with torch.no_grad():
+            return barycentric_weights(tri, bp)
+
+    def bary_fwd_bwd():
+        pg = bp.detach().requires_grad_(True)
+        vg = tri.d_vertices.detach().requires_grad_(True)
+        _, w = barycentric_weights(tri, pg, vg)
+        w.backward(Gb)
+        return pg.grad, vg.grad
+
+    for name, fn in (("barycentric_1e6_fwd_ms", bary_fwd), ("barycentric_1e6_fwd_bwd_ms", bary_fwd_bwd),
+                     ("compute_barycentric_weights_1e6_ms", lambda: tri.compute_barycentric_weights(bp))):
+        fn()
+        wall, _ = time_calls(fn, 10)
+        out[name] = 1e3 * wall / 10
+    del tri, bp, Gb
     sv, sf, tv, tf, _ = S.config3()
     for label in ("build_999698_tris_cold_ms", "build_999698_tris_ms"):
         t0 = time.perf_counter()

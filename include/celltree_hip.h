@@ -433,6 +433,15 @@ int ct_barycentric_triangle_backward(const double* points, const int64_t* face_i
                                      const double* vertices, const double* g_weights,
                                      double* g_points, double* g_vertices, void* stream);
 
+/* Adjoint of barycentric_wachspress.py:38-85 with each point's face held fixed, and the
+ * forward's branch held fixed (interior weights, or the linear interpolation along the first
+ * edge within `tol`): for dL/dw (n, n_max_vert) writes dL/dpoints (n, 2) (NULL to skip) and
+ * adds dL/dvertices (NULL to skip; the caller zeroes it).  3 <= n_max_vert <= 16.  Points with
+ * face -1 get zero gradient. */
+int ct_barycentric_wachspress_backward(const double* points, const int64_t* face_indices, int64_t n,
+                                       const int32_t* faces, int32_t n_max_vert, const double* vertices, double tol,
+                                       const double* g_weights, double* g_points, double* g_vertices, void* stream);
+
 /* ------------------------------------------------------------------ batched trees */
 /* No reference counterpart: many trees and their query sets in one persistent launch,
  * replacing a loop of per-mesh calls (celltree.py:216-239 locate_points, celltree.py:294-303

@@ -153,6 +153,7 @@ SIGNATURES = {
     "ct_locate_points_in_order": (ctypes.c_int, [_TP, _P, _I64, _P, _P, _F64, _P, _P]),
     "ct_check_tree": (ctypes.c_int, [_P, _I64, _P, _P, _I64, _I32, _I64, _I32, _P]),
     "ct_node_parents": (ctypes.c_int, [_TP, _P, _P]),
+    "ct_barycentric_wachspress_backward": (ctypes.c_int, [_P, _P, _I64, _P, _I32, _P, _F64, _P, _P, _P, _P]),
     "ct_locate_points_ordered_noalloc": (ctypes.c_int, [_TP, _P, _I64, _F64, _P, _P]),
     "ct_locate_points_on_edge": (ctypes.c_int, [_TP, _P, _I64, _F64, _P, _P]),
     "ct_order_workspace_bytes": (_SZ, [_I64]),

@@ -585,6 +585,204 @@ __global__ void __launch_bounds__(kBlock) k_bary_tri_backward(const double* __re
   }
 }
 
+// Adjoint of barycentric_wachspress.py:38-85 compute_weights (face and branch held fixed).
+// With E_k = cross(v_{k+1} - v_k, p - v_k) (edge k, v_n = v_0) and
+// X_i = cross(v_i - v_{i-1}, v_{i+1} - v_{i-1}):
+//   interior: w_i = 2 |X_i| / (|E_{i-1}| |E_i|), lambda_i = w_i / sum_j w_j;
+//   edge case (p within tol of edge k, the first such edge in the forward's order):
+//   lambda_k = 1 - t, lambda_{k+1} = t, t = |p - v_k| / |v_{k+1} - v_k|.
+// The forward quantities are recomputed with the forward's own expressions, so the branch
+// (interior or which edge) is the one the forward took.  Vertex gradients are added with
+// atomics (a vertex is shared by many faces).
+template <int MAXV>
+__global__ void __launch_bounds__(kBlock) k_bary_wachspress_backward(
+    const double* __restrict__ pts, const int64_t* __restrict__ face, int64_t n, const int32_t* __restrict__ faces,
+    int nmax, const double* __restrict__ vtx, double tol, const double* __restrict__ gw, double* __restrict__ g_pts,
+    double* __restrict__ g_vtx) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int64_t f = face[q];
+  if (f < 0) {
+    if (g_pts) {
+      g_pts[2 * q] = 0.0;
+      g_pts[2 * q + 1] = 0.0;
+    }
+    return;
+  }
+  P2 poly[MAXV];
+  int vid[MAXV];
+  const int len = load_face_idx<MAXV>(faces, nmax, f, vtx, poly, vid);
+  const P2 p = {pts[2 * q], pts[2 * q + 1]};
+  const double* g = gw + q * nmax;
+  P2 last = poly[0];
+#pragma unroll
+  for (int k = 0; k < MAXV; ++k)
+    if (k == len - 1) last = poly[k];
+  P2 gv[MAXV];
+#pragma unroll
+  for (int k = 0; k < MAXV; ++k) gv[k] = P2{0.0, 0.0};
+  P2 gv_last = {0.0, 0.0};  // added to vertex len - 1 at the end
+  double gpx = 0.0, gpy = 0.0;
+  // Edge-case adjoint on edge (a -> b): t = |V| / |U|, V = p - a, U = b - a.
+  auto edge_adjoint = [&](P2 a, P2 b, double gt, P2* ga, P2* gb) {
+    const double vx = p.x - a.x, vy = p.y - a.y, ux = b.x - a.x, uy = b.y - a.y;
+    const double nv = sqrt(dot(vx, vy, vx, vy)), nu = sqrt(dot(ux, uy, ux, uy));
+    double tvx = 0.0, tvy = 0.0;  // dt/dV (0 at V = 0, where |V| has no derivative)
+    if (nv > 0.0) {
+      tvx = vx / (nv * nu);
+      tvy = vy / (nv * nu);
+    }
+    const double s = nv / (nu * nu * nu);
+    const double tux = -s * ux, tuy = -s * uy;  // dt/dU
+    gpx += gt * tvx;
+    gpy += gt * tvy;
+    ga->x += -gt * (tvx + tux);
+    ga->y += -gt * (tvy + tuy);
+    gb->x += gt * tux;
+    gb->y += gt * tuy;
+  };
+  // Forward, in the forward's order: the edge case on edge len - 1 first, then per vertex i
+  // X_i and E_i, stopping at the first edge within tolerance.
+  double E[MAXV], X[MAXV];
+  int edge = -1;
+  double ux = poly[0].x - last.x, uy = poly[0].y - last.y;
+  const double e_last = cross(ux, uy, p.x - last.x, p.y - last.y);
+  if (within_perp(fabs(e_last), ux, uy, tol)) edge = len - 1;
+  if (edge < 0) {
+#pragma unroll
+    for (int i = 0; i < MAXV; ++i) {
+      if (i < len && edge < 0) {
+        const P2 a = (i == 0) ? last : poly[i > 0 ? i - 1 : 0];
+        const P2 b = poly[i];
+        const P2 c = (i + 1 == len) ? poly[0] : poly[i + 1 < MAXV ? i + 1 : 0];
+        X[i] = cross(ux, uy, c.x - a.x, c.y - a.y);
+        ux = c.x - b.x;
+        uy = c.y - b.y;
+        E[i] = cross(ux, uy, p.x - b.x, p.y - b.y);
+        if (within_perp(fabs(E[i]), ux, uy, tol)) edge = i;
+      }
+    }
+  }
+  if (edge >= 0) {
+    // lambda_edge = 1 - t, lambda_next = t
+#pragma unroll
+    for (int i = 0; i < MAXV; ++i) {
+      if (i == edge) {
+        const int j = (i + 1 == len) ? 0 : i + 1;
+        const double gt = g[j] - g[i];
+        const P2 a = poly[i];
+        const P2 b = (i + 1 == len) ? poly[0] : poly[i + 1 < MAXV ? i + 1 : 0];
+        P2 ga = {0.0, 0.0}, gb = {0.0, 0.0};
+        edge_adjoint(a, b, gt, &ga, &gb);
+        gv[i].x += ga.x;
+        gv[i].y += ga.y;
+        if (i + 1 == len) {
+          gv[0].x += gb.x;
+          gv[0].y += gb.y;
+        } else if (i + 1 < MAXV) {
+          gv[i + 1].x += gb.x;
+          gv[i + 1].y += gb.y;
+        }
+      }
+    }
+  } else {
+    // interior: w_i = 2 |X_i| / (|E_{i-1}| |E_i|), E_{-1} = E_{len-1} = e_last
+    double w[MAXV], S = 0.0;
+#pragma unroll
+    for (int i = 0; i < MAXV; ++i) {
+      if (i < len) {
+        const double ep = (i == 0) ? e_last : E[i > 0 ? i - 1 : 0];
+        w[i] = 2 * fabs(X[i]) / (fabs(ep) * fabs(E[i]));
+        S += w[i];
+      }
+    }
+    double dotgl = 0.0;
+#pragma unroll
+    for (int i = 0; i < MAXV; ++i)
+      if (i < len) dotgl += g[i] * (w[i] / S);
+    double gE[MAXV];
+#pragma unroll
+    for (int i = 0; i < MAXV; ++i) gE[i] = 0.0;
+    double gE_last = 0.0;  // extra gradient on E_{len-1} from vertex 0
+    auto sgn = [](double x) { return x > 0 ? 1.0 : (x < 0 ? -1.0 : 0.0); };
+#pragma unroll
+    for (int i = 0; i < MAXV; ++i) {
+      if (i < len) {
+        const double gwi = (g[i] - dotgl) / S;
+        const double ep = (i == 0) ? e_last : E[i > 0 ? i - 1 : 0];
+        const double Ap = fabs(ep), Ai = fabs(E[i]);
+        const double gX = gwi * 2.0 / (Ap * Ai) * sgn(X[i]);
+        const double gAp = -gwi * w[i] / Ap, gAi = -gwi * w[i] / Ai;
+        gE[i] += gAi * sgn(E[i]);
+        if (i == 0) gE_last += gAp * sgn(ep);
+        else gE[i > 0 ? i - 1 : 0] += gAp * sgn(ep);
+        // X_i = cross(P, Q), P = v_i - v_{i-1}, Q = v_{i+1} - v_{i-1}
+        const P2 a = (i == 0) ? last : poly[i > 0 ? i - 1 : 0];
+        const P2 b = poly[i];
+        const P2 c = (i + 1 == len) ? poly[0] : poly[i + 1 < MAXV ? i + 1 : 0];
+        const double Px = b.x - a.x, Py = b.y - a.y, Qx = c.x - a.x, Qy = c.y - a.y;
+        const double gPx = gX * Qy, gPy = -gX * Qx, gQx = -gX * Py, gQy = gX * Px;
+        gv[i].x += gPx;
+        gv[i].y += gPy;
+        if (i + 1 == len) {
+          gv[0].x += gQx;
+          gv[0].y += gQy;
+        } else if (i + 1 < MAXV) {
+          gv[i + 1].x += gQx;
+          gv[i + 1].y += gQy;
+        }
+        if (i == 0) {
+          gv_last.x -= gPx + gQx;
+          gv_last.y -= gPy + gQy;
+        } else {
+          gv[i > 0 ? i - 1 : 0].x -= gPx + gQx;
+          gv[i > 0 ? i - 1 : 0].y -= gPy + gQy;
+        }
+      }
+    }
+    // E_k = cross(U, V), U = v_{k+1} - v_k, V = p - v_k
+#pragma unroll
+    for (int k = 0; k < MAXV; ++k) {
+      if (k < len) {
+        const double ge = gE[k] + (k == len - 1 ? gE_last : 0.0);
+        const P2 a = poly[k];
+        const P2 b = (k + 1 == len) ? poly[0] : poly[k + 1 < MAXV ? k + 1 : 0];
+        const double Ux = b.x - a.x, Uy = b.y - a.y, Vx = p.x - a.x, Vy = p.y - a.y;
+        const double gUx = ge * Vy, gUy = -ge * Vx, gVx = -ge * Uy, gVy = ge * Ux;
+        gpx += gVx;
+        gpy += gVy;
+        gv[k].x -= gUx + gVx;
+        gv[k].y -= gUy + gVy;
+        if (k + 1 == len) {
+          gv[0].x += gUx;
+          gv[0].y += gUy;
+        } else if (k + 1 < MAXV) {
+          gv[k + 1].x += gUx;
+          gv[k + 1].y += gUy;
+        }
+      }
+    }
+  }
+  if (g_pts) {
+    g_pts[2 * q] = gpx;
+    g_pts[2 * q + 1] = gpy;
+  }
+  if (g_vtx) {
+#pragma unroll
+    for (int k = 0; k < MAXV; ++k) {
+      if (k < len) {
+        double gx = gv[k].x, gy = gv[k].y;
+        if (k == len - 1) {
+          gx += gv_last.x;
+          gy += gv_last.y;
+        }
+        atomicAdd(g_vtx + 2 * vid[k], gx);
+        atomicAdd(g_vtx + 2 * vid[k] + 1, gy);
+      }
+    }
+  }
+}
+
 inline int vclass(int nmax) { return nmax <= 4 ? 4 : nmax <= 8 ? 8 : 16; }
 
 #define CT_PAIR_DISPATCH(K, A, B, ...)                                                       \
@@ -672,6 +870,26 @@ int ct_barycentric_triangle_backward(const double* points, const int64_t* face_i
   if (n == 0) return CT_OK;
   k_bary_tri_backward<<<grid_for(n), kBlock, 0, as_stream(stream)>>>(points, face_indices, n, faces, n_max_vert,
                                                                      vertices, g_weights, g_points, g_vertices);
+  CT_LAUNCH_CHECK();
+  return CT_OK;
+}
+
+int ct_barycentric_wachspress_backward(const double* points, const int64_t* face_indices, int64_t n,
+                                       const int32_t* faces, int32_t n_max_vert, const double* vertices, double tol,
+                                       const double* g_weights, double* g_points, double* g_vertices, void* stream) {
+  CT_REQUIRE(n >= 0 && n_max_vert >= 3 && n_max_vert <= 16, "n_max_vert must be in [3, 16]");
+  if (n == 0) return CT_OK;
+  const unsigned g = grid_for(n);
+  hipStream_t st = as_stream(stream);
+  if (n_max_vert <= 4)
+    k_bary_wachspress_backward<4><<<g, kBlock, 0, st>>>(points, face_indices, n, faces, n_max_vert, vertices, tol,
+                                                        g_weights, g_points, g_vertices);
+  else if (n_max_vert <= 8)
+    k_bary_wachspress_backward<8><<<g, kBlock, 0, st>>>(points, face_indices, n, faces, n_max_vert, vertices, tol,
+                                                        g_weights, g_points, g_vertices);
+  else
+    k_bary_wachspress_backward<16><<<g, kBlock, 0, st>>>(points, face_indices, n, faces, n_max_vert, vertices, tol,
+                                                         g_weights, g_points, g_vertices);
   CT_LAUNCH_CHECK();
   return CT_OK;
 }

@@ -257,13 +257,13 @@ def test_barycentric_adjoint():
     vv = tree.d_vertices.clone().requires_grad_(True)
     _, w = barycentric_weights(tree, p, vv)
     (w * G).sum().backward()
-    from numba_celltree_amd.barycentric import BarycentricTriangleFunction
+    from numba_celltree_amd.barycentric import BarycentricFunction
 
     tol = tree._default_tolerance()
 
     def loss(px, vx):
         with torch.no_grad():
-            return float((BarycentricTriangleFunction.apply(px, vx, tree.d_elements, face, tol) * G).sum())
+            return float((BarycentricFunction.apply(px, vx, tree.d_elements, face, tol, False) * G).sum())
 
     # fourth-order central differences: truncation O(h^4), rounding ~ eps * |loss| / h
     h = 1e-6
@@ -283,7 +283,7 @@ def test_barycentric_adjoint():
     small = pts[:50].clone().requires_grad_(True)
     vs = tree.d_vertices.clone().requires_grad_(True)
     assert torch.autograd.gradcheck(
-        lambda a, b: BarycentricTriangleFunction.apply(a, b, tree.d_elements, face[:50], tol), (small, vs),
+        lambda a, b: BarycentricFunction.apply(a, b, tree.d_elements, face[:50], tol, False), (small, vs),
         eps=1e-6, atol=1e-8, rtol=1e-6,
     )
 
@@ -295,7 +295,7 @@ def test_inputs_are_checked_or_converted():
     import torch
 
     import numba_celltree_amd as M
-    from numba_celltree_amd.barycentric import BarycentricTriangleFunction, barycentric_weights
+    from numba_celltree_amd.barycentric import BarycentricFunction, barycentric_weights
     from numba_celltree_amd.regrid import RegridFunction
 
     sv, sf, tv, tf, F = problem(21, 8, 4.0, n_fields=3)
@@ -329,7 +329,7 @@ def test_inputs_are_checked_or_converted():
     with pytest.raises(ValueError):
         barycentric_weights(tri, pts, vertices=tri.d_vertices[:-1])
     with pytest.raises(TypeError):
-        BarycentricTriangleFunction.apply(pts.float(), tri.d_vertices, tri.d_elements, face, 1e-12)
+        BarycentricFunction.apply(pts.float(), tri.d_vertices, tri.d_elements, face, 1e-12, False)
 
 
 # ------------------------------------------------------------- operator assembly
