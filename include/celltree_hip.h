@@ -284,6 +284,54 @@ int ct_locate_edges_write_rec(const ct_tree* tree, const double* segments, int64
                               const int32_t* hits, int32_t hit_slots, int64_t* ii, int64_t* jj,
                               double* xy, void* stream);
 
+/* ------------------------------------------------------------------ work splitting
+ * No reference counterpart: the largest box and segment queries (query.py:177-244, 357-455)
+ * walk hundreds of leaves in one lane and set a launch's tail.  A query is cut into
+ * subqueries by subtree: its walk from the root is expanded into at most max_parts viable
+ * subtrees, in the walk's visit order (the subtree with the most cells expanded first, while
+ * it holds more than leaf_cells cells), and each subquery walks one subtree.  The subqueries
+ * of a query, one after the other, emit exactly the query's rows in the query's order.
+ * Queries expected to meet at most min_cells cells (a box: (w / c + 1) (h / c + 1), a segment
+ * 1.5 |b - a| / c + 1, c = sqrt(tree bbox area / n_elements)) stay whole: one subquery at
+ * the root.  Trees of depth <= CT_MAX_DEPTH. */
+typedef struct ct_subquery {
+  int32_t query;  /* query index */
+  int32_t node;   /* subtree root (0: the whole walk, tree-bbox test included) */
+} ct_subquery;
+
+/* kind 1 boxes (n, 4), 2 segment-face, 3 segment-edge (segments (n, 2, 2)); `order` (may be
+ * NULL) lists the queries in processing order (ct_query_order).  Count call (subs NULL):
+ * parts[i] = number of subqueries of query order[i].  Write call: subs[part_offsets[i] + k]
+ * (part_offsets: ct_exclusive_scan of parts), subqueries in processing order.
+ * 1 <= max_parts <= 16; trees of fewer than 2**28 nodes. */
+int ct_split_queries(const ct_tree* tree, int32_t kind, const double* queries, int64_t n, const int32_t* order,
+                     int32_t min_cells, int32_t leaf_cells, int32_t max_parts, int64_t* parts,
+                     const int64_t* part_offsets, ct_subquery* subs, void* stream);
+
+/* After a count pass over the subqueries: counts[j] becomes the exclusive prefix of the
+ * counts of its query's earlier subqueries, and per_query[q] the query's total (the input of
+ * ct_exclusive_scan, whose output is query_offsets). */
+int ct_subquery_offsets(const int32_t* order, int64_t n, const int64_t* part_offsets, int64_t* counts,
+                        int64_t* per_query, void* stream);
+
+/* query.py:278-289 locate_boxes over subqueries: count pass (counts per subquery, hit record
+ * hits[slot * n_sub + j]) and write pass (rows of subquery j at query_offsets[q] + prefix[j],
+ * copied from the record when they fit). */
+int ct_locate_boxes_count_sub(const ct_tree* tree, const double* boxes, const ct_subquery* subs, int64_t n_sub,
+                              int64_t* counts, int32_t* hits, int32_t hit_slots, void* stream);
+int ct_locate_boxes_write_sub(const ct_tree* tree, const double* boxes, const ct_subquery* subs, int64_t n_sub,
+                              const int64_t* query_offsets, const int64_t* prefix, const int32_t* hits,
+                              int32_t hit_slots, int64_t* ii, int64_t* jj, void* stream);
+
+/* query.py:500-544 locate_edge_faces (kind 1) / locate_edge_edges (kind 0) over subqueries,
+ * as the box pair above; rows in DFS order per query (ct_sort_intersections_by_edge then
+ * gives the reference's order). */
+int ct_locate_edges_count_sub(const ct_tree* tree, const double* segments, const ct_subquery* subs, int64_t n_sub,
+                              int32_t kind, int64_t* counts, int32_t* hits, int32_t hit_slots, void* stream);
+int ct_locate_edges_write_sub(const ct_tree* tree, const double* segments, const ct_subquery* subs, int64_t n_sub,
+                              int32_t kind, const int64_t* query_offsets, const int64_t* prefix, const int32_t* hits,
+                              int32_t hit_slots, int64_t* ii, int64_t* jj, double* xy, void* stream);
+
 /* geometry_utils.py:564-574 sort_intersections_by_edge on grouped output (ii ascending, the
  * rows of segment s in [offsets[s], offsets[s+1])): writes jj and xy (n_rows, 2, 2) stably
  * sorted within each segment by t = (xy[:,0] - a) . (b - a), NaN last, into jj_out / xy_out

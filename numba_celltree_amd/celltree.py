@@ -90,6 +90,19 @@ class CellTree2d(CellTree2dBase):
         n = int(d_boxes.shape[0])
         st = D.stream()
         order, srt = self._order(1, d_boxes)
+        split = self._split(1, d_boxes, order)
+        if split is not None:
+            subs, part_offsets, m = split
+            counts = D.empty(max(m, 1), torch.int64)
+            hits, slots = self._hit_record(m, self.BOX_HIT_SLOTS, self.BOX_HIT_BUDGET)
+            _native.call("ct_locate_boxes_count_sub", self.tree_ref, D.ptr(d_boxes), D.ptr(subs), m, D.ptr(counts),
+                         D.ptr(hits), slots, st)
+            offsets, total = self._sub_offsets(order, n, part_offsets, counts)
+            ii = D.empty(total, torch.int64)
+            jj = D.empty(total, torch.int64)
+            _native.call("ct_locate_boxes_write_sub", self.tree_ref, D.ptr(d_boxes), D.ptr(subs), m, D.ptr(offsets),
+                         D.ptr(counts), D.ptr(hits), slots, D.ptr(ii), D.ptr(jj), st)
+            return ii, jj
         counts = D.empty(n, torch.int64)
         hits, slots = self._hit_record(n, self.BOX_HIT_SLOTS, self.BOX_HIT_BUDGET)
         _native.call("ct_locate_boxes_count_rec", self.tree_ref, D.ptr(d_boxes), n, D.ptr(order), D.ptr(srt),

@@ -390,12 +390,66 @@ class CellTree2dBase:
                 slots //= 2
         return None, 0
 
+    # Work splitting of the largest box and segment queries (ct_split_queries): a query expected
+    # to meet more than SPLIT_MIN_CELLS cells is cut into at most SPLIT_MAX_PARTS subqueries by
+    # subtree, expanding subtrees of more than SPLIT_LEAF_CELLS cells; results are identical.
+    SPLIT_MIN_CELLS = 16
+    SPLIT_LEAF_CELLS = 16
+    SPLIT_MAX_PARTS = 16
+
+    def _split(self, kind: int, queries: torch.Tensor, order):
+        """(subqueries (m, 2) int32, part offsets (n + 1), m) for kind 1 boxes, 2 segment-face,
+        3 segment-edge, or None when the batch is not split."""
+        n = int(queries.shape[0])
+        if order is None or self.depth > _native.CT_MAX_DEPTH or int(self.SPLIT_MAX_PARTS) <= 1:
+            return None
+        st = D.stream()
+        args = (self.tree_ref, kind, D.ptr(queries), n, D.ptr(order), int(self.SPLIT_MIN_CELLS),
+                int(self.SPLIT_LEAF_CELLS), int(self.SPLIT_MAX_PARTS))
+        parts = D.empty(n, torch.int64)
+        _native.call("ct_split_queries", *args, D.ptr(parts), None, None, st)
+        part_offsets, m = D.exclusive_scan(parts)
+        subs = D.empty((max(m, 1), 2), torch.int32)
+        _native.call("ct_split_queries", *args, None, D.ptr(part_offsets), D.ptr(subs), st)
+        return subs, part_offsets, m
+
+    def _sub_offsets(self, order, n: int, part_offsets, counts):
+        """Turns per-subquery counts into prefixes within each query; returns the query offsets
+        (n + 1) and the row total."""
+        per_query = D.empty(n, torch.int64)
+        _native.call("ct_subquery_offsets", D.ptr(order), n, D.ptr(part_offsets), D.ptr(counts), D.ptr(per_query),
+                     D.stream())
+        return D.exclusive_scan(per_query)
+
+    def _sort_edges(self, edge_coords, offsets, ii, jj, xy, total):
+        jj_sorted = torch.empty_like(jj)
+        xy_sorted = torch.empty_like(xy)
+        _native.call(
+            "ct_sort_intersections_by_edge", D.ptr(edge_coords), D.ptr(offsets), D.ptr(ii), D.ptr(jj),
+            D.ptr(xy), total, D.ptr(jj_sorted), D.ptr(xy_sorted), D.stream(),
+        )
+        return ii, jj_sorted, xy_sorted
+
     def _locate_edges(self, edge_coords, kind: int):
         """count (recording hits) -> scan -> write, then the per-segment sort by t (device
         tensors)."""
         n = int(edge_coords.shape[0])
         st = D.stream()
         order, srt = self._order(2, edge_coords)
+        split = self._split(2 if kind == 1 else 3, edge_coords, order)
+        if split is not None:
+            subs, part_offsets, m = split
+            counts = D.empty(max(m, 1), torch.int64)
+            hits, slots = self._hit_record(m, self.EDGE_HIT_SLOTS, self.EDGE_HIT_BUDGET)
+            _native.call("ct_locate_edges_count_sub", self.tree_ref, D.ptr(edge_coords), D.ptr(subs), m, kind,
+                         D.ptr(counts), D.ptr(hits), slots, st)
+            offsets, total = self._sub_offsets(order, n, part_offsets, counts)
+            ii = D.empty(total, torch.int64)
+            jj = D.empty(total, torch.int64)
+            xy = D.empty((total, 2, 2), torch.float64)
+            _native.call("ct_locate_edges_write_sub", self.tree_ref, D.ptr(edge_coords), D.ptr(subs), m, kind,
+                         D.ptr(offsets), D.ptr(counts), D.ptr(hits), slots, D.ptr(ii), D.ptr(jj), D.ptr(xy), st)
+            return self._sort_edges(edge_coords, offsets, ii, jj, xy, total)
         counts = D.empty(n, torch.int64)
         hits, slots = self._hit_record(n, self.EDGE_HIT_SLOTS, self.EDGE_HIT_BUDGET)
         _native.call(
@@ -410,10 +464,4 @@ class CellTree2dBase:
             "ct_locate_edges_write_rec", self.tree_ref, D.ptr(edge_coords), n, kind, D.ptr(order), D.ptr(srt),
             D.ptr(offsets), D.ptr(hits), slots, D.ptr(ii), D.ptr(jj), D.ptr(xy), st,
         )
-        jj_sorted = torch.empty_like(jj)
-        xy_sorted = torch.empty_like(xy)
-        _native.call(
-            "ct_sort_intersections_by_edge", D.ptr(edge_coords), D.ptr(offsets), D.ptr(ii), D.ptr(jj),
-            D.ptr(xy), total, D.ptr(jj_sorted), D.ptr(xy_sorted), st,
-        )
-        return ii, jj_sorted, xy_sorted
+        return self._sort_edges(edge_coords, offsets, ii, jj, xy, total)

@@ -97,7 +97,9 @@ __device__ __forceinline__ unsigned order_key(int kind, const double* __restrict
     w = fabs(d.z - d.x);
     h = fabs(d.w - d.y);
   }
-  const double rel = (w * h) * g.inv_area;  // fraction of the tree bbox covered
+  // Work proxy relative to the tree bbox: a box's area (its hits grow with it); a segment's
+  // squared length (its walk grows with its length; its bbox area is 0 when axis-parallel).
+  const double rel = (kind == 1 ? w * h : w * w + h * h) * g.inv_area;
   int cls;                                   // 2^-24 .. 2^-9 -> 0 .. 15
   if (!(rel > 0.0)) {
     cls = (rel == rel) ? 0 : 15;  // empty -> 0, NaN -> 15
@@ -109,7 +111,9 @@ __device__ __forceinline__ unsigned order_key(int kind, const double* __restrict
   }
   const unsigned m = spread_bits(grid_coord(cx, g.bb.xmin, g.sx, 256.0)) |
                      (spread_bits(grid_coord(cy, g.bb.ymin, g.sy, 256.0)) << 1);
-  return ((unsigned)cls << 16) | (m & 0xffffu);
+  // The heaviest class first: workgroups start roughly in order, so the longest walks start
+  // early and the light ones fill in behind them (longest-processing-time first).
+  return ((unsigned)(15 - cls) << 16) | (m & 0xffffu);
 }
 
 // Two-level bucket sort on the 20-bit key, without global atomics:

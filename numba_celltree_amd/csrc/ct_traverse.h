@@ -339,6 +339,51 @@ __device__ __forceinline__ bool seg_meets_box(P2 a, double dx, double dy, double
   return !(t0 > t1);
 }
 
+// Which children of internal node nd segment (a, b) (v = b - a) may reach: query.py:407-441's
+// slab test on the split dimension (the t-tests always hold once the slab test passed, but
+// are kept), then, when nbox is set (tight node boxes, the segment finite), seg_meets_box.
+__device__ __forceinline__ void seg_children(const NodeLd& nd, P2 a, P2 b, double vx, double vy, double mseg,
+                                             const double4* __restrict__ nbox, bool* left_out, bool* right_out) {
+  const double dx = nd.dim ? vy : vx;
+  const double an = nd.dim ? a.y : a.x;
+  const double bn = nd.dim ? b.y : b.x;
+  double dl, dr;
+  if (dx > 0.0) {
+    dl = nd.Lmax - an;
+    dr = nd.Rmin - bn;
+  } else {
+    dl = nd.Lmax - bn;
+    dr = nd.Rmin - an;
+  }
+  bool left = dl >= 0.0;
+  bool right = dr <= 0.0;
+  if (dx > 0.0) {
+    if (left) left = (dl / dx) >= 0.0;
+    if (right) right = (dr / dx) <= 1.0;
+  } else if (dx < 0.0) {
+    if (left) left = (1.0 - (dl / dx)) >= 0.0;
+    if (right) right = (1.0 - (dr / dx)) <= 1.0;
+  }
+  if (nbox != nullptr && (left || right)) {
+    // the two children's boxes are rows child + 1, child + 2: one aligned 64-byte pair
+    if (left) left = seg_meets_box(a, vx, vy, mseg, nbox[nd.child]);
+    if (right) right = seg_meets_box(a, vx, vy, mseg, nbox[nd.child + 1]);
+  }
+  *left_out = left;
+  *right_out = right;
+}
+
+// Per-segment setup of the subtree pruning: the node-box rows (or null when the tree has none,
+// the query is a segment-edge query, or the segment is not finite) and the segment's largest
+// coordinate magnitude.
+template <bool FACES>
+__device__ __forceinline__ const double4* seg_prune_boxes(const TreeView& t, P2 a, P2 b, double vx, double vy,
+                                                          double* mseg) {
+  *mseg = pmax(pmax(fabs(a.x), fabs(a.y)), pmax(fabs(b.x), fabs(b.y)));
+  const bool prune = FACES && t.nbox != nullptr && isfinite(*mseg) && isfinite(vx) && isfinite(vy);
+  return prune ? reinterpret_cast<const double4*>(t.nbox) + 1 : nullptr;
+}
+
 // The first `slots` hit cells of one query, kept by a count pass so that the write pass can
 // replay them instead of walking the tree again: hit c at cells[c * stride].
 struct HitRecord {
@@ -352,52 +397,30 @@ struct HitRecord {
 // passed, but are kept).  Returns the number of hits; with WRITE, hit rows go to o, o+1, ...;
 // without, the first rec.slots hit cells go to rec (when rec.cells is set).
 // Segment-face queries on a tree with node boxes also skip subtrees by seg_meets_box.
+// `start` != 0: walk only the subtree of node `start`, which the caller reached by this same
+// walk's tests (a subquery, see ct_split.hip); the tree-bbox test was passed then.
 template <bool WRITE, int MAXV, int STACK, bool FACES>
 __device__ __forceinline__ int64_t edge_query(const TreeView& t, P2 a, P2 b, int64_t q, int64_t o,
                                               int64_t* __restrict__ ii, int64_t* __restrict__ jj,
                                               double* __restrict__ xy, LaneStack<STACK>& stack,
-                                              HitRecord rec = HitRecord{}) {
+                                              HitRecord rec = HitRecord{}, int start = 0) {
   int64_t cnt = 0;
   P2 c0, d0;
-  if (!cohen_sutherland(a, b, t.bbox, &c0, &d0)) return 0;
+  if (start == 0 && !cohen_sutherland(a, b, t.bbox, &c0, &d0)) return 0;
   const double vx = b.x - a.x, vy = b.y - a.y;
-  const double mseg = pmax(pmax(fabs(a.x), fabs(a.y)), pmax(fabs(b.x), fabs(b.y)));
-  // Non-finite segments keep the reference traversal unchanged.
-  const bool prune = FACES && t.nbox != nullptr && isfinite(mseg) && isfinite(vx) && isfinite(vy);
-  const double4* __restrict__ nbox = prune ? reinterpret_cast<const double4*>(t.nbox) + 1 : nullptr;
+  // Non-finite segments keep the reference traversal unchanged (no pruning).
+  double mseg;
+  const double4* __restrict__ nbox = seg_prune_boxes<FACES>(t, a, b, vx, vy, &mseg);
   // Same loop shape as locate_one: internal nodes in an inner loop, the leaf's cell tests
   // once every lane of the wavefront stands on a leaf or has finished.
   int sp = 0;
-  int node = 0;
+  int node = start;
   while (true) {
     NodeLd nd = ld_node(t.nodes, node);
     bool exhausted = false;
     while (nd.child != -1) {
-      const double dx = nd.dim ? vy : vx;
-      const double an = nd.dim ? a.y : a.x;
-      const double bn = nd.dim ? b.y : b.x;
-      double dl, dr;
-      if (dx > 0.0) {
-        dl = nd.Lmax - an;
-        dr = nd.Rmin - bn;
-      } else {
-        dl = nd.Lmax - bn;
-        dr = nd.Rmin - an;
-      }
-      bool left = dl >= 0.0;
-      bool right = dr <= 0.0;
-      if (dx > 0.0) {
-        if (left) left = (dl / dx) >= 0.0;
-        if (right) right = (dr / dx) <= 1.0;
-      } else if (dx < 0.0) {
-        if (left) left = (1.0 - (dl / dx)) >= 0.0;
-        if (right) right = (1.0 - (dr / dx)) <= 1.0;
-      }
-      if (prune && (left || right)) {
-        // the two children's boxes are rows child + 1, child + 2: one aligned 64-byte pair
-        if (left) left = seg_meets_box(a, vx, vy, mseg, nbox[nd.child]);
-        if (right) right = seg_meets_box(a, vx, vy, mseg, nbox[nd.child + 1]);
-      }
+      bool left, right;
+      seg_children(nd, a, b, vx, vy, mseg, nbox, &left, &right);
       if (left && right) {
         stack.put(sp++, nd.child);
         node = nd.child + 1;

@@ -690,3 +690,85 @@ def test_locate_boxes_hit_record_is_exact(trees):
     for slots in (1, 5, 32):
         c, i, j = run(slots)
         assert np.array_equal(c, c0) and np.array_equal(i, i0) and np.array_equal(j, j0), slots
+
+
+@pytest.mark.parametrize("max_parts, leaf_cells, slots", [(2, 2, 4), (7, 4, 0), (16, 1, 64), (16, 16, 1024)])
+def test_split_queries_are_exact(trees, max_parts, leaf_cells, slots):
+    """Box and segment queries cut into subqueries by subtree (ct_split_queries) give the
+    rows of the whole walks, bit for bit and in the same order, and the oracle's: every
+    query is split (no size threshold), with several frontier sizes and hit records from
+    none to every row, so both the replayed and the re-walked write paths run; queries from a
+    fraction of a cell to across the mesh, some outside it, some along mesh edges."""
+    name, v, f, g, o = trees
+    L = _extent(v)
+    lo = v.min(0)
+    h = L / np.sqrt(len(f))
+    rng = np.random.default_rng(31)
+    n = 5000
+    c = lo + rng.uniform(-0.05 * L, 1.05 * L, size=(n, 2))
+    wh = L * np.exp(rng.uniform(np.log(1e-4), np.log(0.3), size=(n, 2)))
+    boxes = np.column_stack([c[:, 0] - wh[:, 0], c[:, 0] + wh[:, 0], c[:, 1] - wh[:, 1], c[:, 1] + wh[:, 1]])
+    length = L * np.exp(rng.uniform(np.log(1e-4), 0.0, size=(n, 1)))
+    ang = rng.uniform(0, 2 * np.pi, size=(n, 1))
+    seg = np.stack([c, c + length * np.concatenate([np.cos(ang), np.sin(ang)], axis=1)], axis=1)
+    fv = f[:, :3]
+    along = np.stack([v[fv[:300, 0]], v[fv[:300, 1]]], axis=1)
+    seg = np.concatenate([seg, along, along[:, ::-1] + np.array([0.0, 0.3 * h])])
+    log(f"split queries {name} parts={max_parts} leaf={leaf_cells} slots={slots}")
+
+    def run(split: bool):
+        g.ORDER_MIN = 0  # the split path runs on ordered batches
+        g.SPLIT_MIN_CELLS = 0 if split else 10**9
+        g.SPLIT_MAX_PARTS = max_parts if split else 1
+        g.SPLIT_LEAF_CELLS = leaf_cells
+        g.BOX_HIT_SLOTS = g.EDGE_HIT_SLOTS = max(slots, 1)
+        g.BOX_HIT_BUDGET = g.EDGE_HIT_BUDGET = (slots * 10**6) if slots >= 4 else 0
+        try:
+            return g.locate_boxes(boxes), g.intersect_edges(seg)
+        finally:
+            for attr in ("ORDER_MIN", "SPLIT_MIN_CELLS", "SPLIT_MAX_PARTS", "SPLIT_LEAF_CELLS", "BOX_HIT_SLOTS",
+                         "EDGE_HIT_SLOTS", "BOX_HIT_BUDGET", "EDGE_HIT_BUDGET"):
+                delattr(g, attr)
+
+    (bi, bj), (ei, ej, exy) = run(True)
+    (ui, uj), (fi, fj, fxy) = run(False)
+    assert np.array_equal(bi, ui) and np.array_equal(bj, uj)
+    assert np.array_equal(ei, fi) and np.array_equal(ej, fj)
+    assert np.array_equal(exy.view(np.int64), fxy.view(np.int64))
+    oi, oj = o.locate_boxes(boxes)
+    assert np.array_equal(bi, oi) and np.array_equal(bj, oj)
+    oi, oj, oxy = o.intersect_edges(seg)
+    assert np.array_equal(ei, oi) and np.array_equal(ej, oj)
+    assert_close(exy, oxy, L)
+    # the split really happened: more subqueries than queries
+    from numba_celltree_amd import _device as D
+
+    import torch
+
+    d = torch.from_numpy(boxes).cuda()
+    g.SPLIT_MIN_CELLS, g.SPLIT_MAX_PARTS, g.SPLIT_LEAF_CELLS = 0, max_parts, leaf_cells
+    try:
+        _, _, m = g._split(1, d, D.query_order(1, d, g._bbox))
+    finally:
+        del g.SPLIT_MIN_CELLS, g.SPLIT_MAX_PARTS, g.SPLIT_LEAF_CELLS
+    assert m > n
+
+
+def test_split_edge_tree_is_exact():
+    """Segment-edge queries of an EdgeCellTree2d split into subqueries equal the whole walks
+    and the oracle."""
+    import numba_celltree_amd as M
+
+    v, f = S.jittered_grid(81, 100.0, 0.25, 1.0, 8)
+    e = np.unique(np.sort(np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]]), axis=1), axis=0)
+    g = M.EdgeCellTree2d(v, e)
+    o = R.OracleEdgeCellTree2d(v, e)
+    rng = np.random.default_rng(5)
+    a = rng.uniform(-5, 105, size=(6000, 2))
+    b = a + 60 * np.exp(rng.uniform(np.log(1e-3), 0, size=(6000, 1))) * rng.normal(size=(6000, 2))
+    seg = np.stack([a, b], axis=1)
+    g.ORDER_MIN, g.SPLIT_MIN_CELLS, g.SPLIT_MAX_PARTS, g.SPLIT_LEAF_CELLS = 0, 0, 12, 2
+    gi, gj, gxy = g.intersect_edges(seg)
+    oi, oj, oxy = o.intersect_edges(seg)
+    assert np.array_equal(gi, oi) and np.array_equal(gj, oj)
+    assert_close(gxy, oxy, 100.0)
