@@ -303,10 +303,12 @@ typedef struct ct_subquery {
  * NULL) lists the queries in processing order (ct_query_order).  Count call (subs NULL):
  * parts[i] = number of subqueries of query order[i].  Write call: subs[part_offsets[i] + k]
  * (part_offsets: ct_exclusive_scan of parts), subqueries in processing order.
- * 1 <= max_parts <= 16; trees of fewer than 2**28 nodes. */
+ * frontier_buf (may be NULL; device int32, n * max_parts): the count call keeps the frontiers
+ * there and the write call copies them instead of computing them again; both calls must get
+ * the same buffer.  1 <= max_parts <= 16; trees of fewer than 2**28 nodes. */
 int ct_split_queries(const ct_tree* tree, int32_t kind, const double* queries, int64_t n, const int32_t* order,
                      int32_t min_cells, int32_t leaf_cells, int32_t max_parts, int64_t* parts,
-                     const int64_t* part_offsets, ct_subquery* subs, void* stream);
+                     const int64_t* part_offsets, ct_subquery* subs, int32_t* frontier_buf, void* stream);
 
 /* After a count pass over the subqueries: counts[j] becomes the exclusive prefix of the
  * counts of its query's earlier subqueries, and per_query[q] the query's total (the input of

@@ -154,7 +154,7 @@ SIGNATURES = {
     "ct_check_tree": (ctypes.c_int, [_P, _I64, _P, _P, _I64, _I32, _I64, _I32, _P]),
     "ct_node_parents": (ctypes.c_int, [_TP, _P, _P]),
     "ct_barycentric_wachspress_backward": (ctypes.c_int, [_P, _P, _I64, _P, _I32, _P, _F64, _P, _P, _P, _P]),
-    "ct_split_queries": (ctypes.c_int, [_TP, _I32, _P, _I64, _P, _I32, _I32, _I32, _P, _P, _P, _P]),
+    "ct_split_queries": (ctypes.c_int, [_TP, _I32, _P, _I64, _P, _I32, _I32, _I32, _P, _P, _P, _P, _P]),
     "ct_subquery_offsets": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P]),
     "ct_locate_boxes_count_sub": (ctypes.c_int, [_TP, _P, _P, _I64, _P, _P, _I32, _P]),
     "ct_locate_boxes_write_sub": (ctypes.c_int, [_TP, _P, _P, _I64, _P, _P, _P, _I32, _P, _P, _P]),

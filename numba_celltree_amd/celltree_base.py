@@ -407,10 +407,11 @@ class CellTree2dBase:
         args = (self.tree_ref, kind, D.ptr(queries), n, D.ptr(order), int(self.SPLIT_MIN_CELLS),
                 int(self.SPLIT_LEAF_CELLS), int(self.SPLIT_MAX_PARTS))
         parts = D.empty(n, torch.int64)
-        _native.call("ct_split_queries", *args, D.ptr(parts), None, None, st)
+        keep = D.empty(n * int(self.SPLIT_MAX_PARTS), torch.int32)  # frontiers, for the write call
+        _native.call("ct_split_queries", *args, D.ptr(parts), None, None, D.ptr(keep), st)
         part_offsets, m = D.exclusive_scan(parts)
         subs = D.empty((max(m, 1), 2), torch.int32)
-        _native.call("ct_split_queries", *args, None, D.ptr(part_offsets), D.ptr(subs), st)
+        _native.call("ct_split_queries", *args, None, D.ptr(part_offsets), D.ptr(subs), D.ptr(keep), st)
         return subs, part_offsets, m
 
     def _sub_offsets(self, order, n: int, part_offsets, counts):

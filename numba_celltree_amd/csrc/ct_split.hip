@@ -104,11 +104,11 @@ __device__ __forceinline__ int frontier_entry(const ct_node* __restrict__ nodes,
 //
 // Expansion runs in rounds: every expandable entry is replaced by its viable children (in
 // visit order: right, then left) at once, so a round's node loads are independent of each
-// other; a round that would exceed max_parts entries is not taken.  codes[k] keeps entry k's
-// expansion in the round: 0 = kept, else (child << 3) | 4 | (left << 1) | right.
+// other; a round that would exceed max_parts entries is not taken.  The second pass of a
+// round tests an expanded entry's children again (its node record is in the cache by then)
+// rather than keeping the first pass's answer in LDS.
 template <bool FACES>
-__device__ int frontier(const TreeView& t, const SplitQuery& s, const SplitParams& p, int* a, int* b, int* codes,
-                        int** out) {
+__device__ int frontier(const TreeView& t, const SplitQuery& s, const SplitParams& p, int* a, int* b, int** out) {
   a[0] = 0;
   *out = a;
   if (!worth_splitting(s, p)) return 1;
@@ -118,7 +118,19 @@ __device__ int frontier(const TreeView& t, const SplitQuery& s, const SplitParam
     P2 c0, d0;
     if (!cohen_sutherland(s.a, s.b, t.bbox, &c0, &d0)) return 0;
   }
-  a[0] = frontier_entry(t.nodes, 0, p.leaf_cells);
+  // The top of the walk: while one child alone is viable, a round would only replace the
+  // single entry by that child; follow the chain directly (one load per level).
+  int node = 0;
+  while (true) {
+    const NodeLd nd = ld_node(t.nodes, node);
+    if (nd.child == -1 || nd.size <= p.leaf_cells) break;
+    bool left, right;
+    children<FACES>(s, nd, &left, &right);
+    if (left && right) break;
+    if (!left && !right) return 0;
+    node = nd.child + (right ? 1 : 0);
+  }
+  a[0] = frontier_entry(t.nodes, node, p.leaf_cells);
   int* cur = a;
   int* nxt = b;
   int m = 1;
@@ -127,28 +139,28 @@ __device__ int frontier(const TreeView& t, const SplitQuery& s, const SplitParam
     bool any = false;
     for (int k = 0; k < m; ++k) {
       const int e = cur[k * kBlock];
-      int code = 0, cnt = 1;
+      int cnt = 1;
       if (e & kExpand) {
         const NodeLd nd = ld_node(t.nodes, e & kNodeMask);
         bool left, right;
         children<FACES>(s, nd, &left, &right);
-        code = (nd.child << 3) | 4 | (left ? 2 : 0) | (right ? 1 : 0);
         cnt = (left ? 1 : 0) + (right ? 1 : 0);
         any = true;
       }
-      codes[k * kBlock] = code;
       m2 += cnt;
     }
     if (!any || m2 > p.max_parts) break;
     int w = 0;
     for (int k = 0; k < m; ++k) {
-      const int code = codes[k * kBlock];
-      if (code == 0) {
-        nxt[w++ * kBlock] = cur[k * kBlock];
+      const int e = cur[k * kBlock];
+      if (!(e & kExpand)) {
+        nxt[w++ * kBlock] = e;
       } else {
-        const int child = code >> 3;
-        if (code & 1) nxt[w++ * kBlock] = frontier_entry(t.nodes, child + 1, p.leaf_cells);
-        if (code & 2) nxt[w++ * kBlock] = frontier_entry(t.nodes, child, p.leaf_cells);
+        const NodeLd nd = ld_node(t.nodes, e & kNodeMask);
+        bool left, right;
+        children<FACES>(s, nd, &left, &right);
+        if (right) nxt[w++ * kBlock] = frontier_entry(t.nodes, nd.child + 1, p.leaf_cells);
+        if (left) nxt[w++ * kBlock] = frontier_entry(t.nodes, nd.child, p.leaf_cells);
       }
     }
     int* tmp = cur;
@@ -161,24 +173,33 @@ __device__ int frontier(const TreeView& t, const SplitQuery& s, const SplitParam
   return m;
 }
 
+// Count call (subs null): parts[i] = m and, when `keep` is set, the frontier nodes in
+// keep[i * max_parts + k].  Write call: the subqueries, from `keep` when set, else computed
+// again (the same frontier).
 template <bool FACES>
 __global__ void __launch_bounds__(kBlock) k_split(TreeView t, int kind, const double* __restrict__ qs,
                                                   const int32_t* __restrict__ order, int64_t n, SplitParams p,
                                                   int64_t* __restrict__ parts, const int64_t* __restrict__ part_offsets,
-                                                  ct_subquery* __restrict__ subs) {
-  // per lane: two frontier buffers and a round's expansion codes (LDS, lane-interleaved)
-  __shared__ int lds[3 * kMaxParts * kBlock];
+                                                  ct_subquery* __restrict__ subs, int32_t* __restrict__ keep) {
+  // per lane: two frontier buffers (LDS, lane-interleaved)
+  __shared__ int lds[2 * kMaxParts * kBlock];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const int64_t q = query_index(order, i);
+  if (subs != nullptr && keep != nullptr) {
+    const int64_t o = part_offsets[i], m = part_offsets[i + 1] - o;
+    for (int64_t k = 0; k < m; ++k) subs[o + k] = ct_subquery{(int32_t)q, keep[i * p.max_parts + k]};
+    return;
+  }
   int* a = lds + threadIdx.x;
   int* b = a + kMaxParts * kBlock;
-  int* codes = b + kMaxParts * kBlock;
-  const int64_t q = query_index(order, i);
   const SplitQuery s = load_split_query<FACES>(t, kind, qs, q);
   int* fr;
-  const int m = frontier<FACES>(t, s, p, a, b, codes, &fr);
+  const int m = frontier<FACES>(t, s, p, a, b, &fr);
   if (subs == nullptr) {
     parts[i] = m;
+    if (keep != nullptr)
+      for (int k = 0; k < m; ++k) keep[i * p.max_parts + k] = fr[k * kBlock] & kNodeMask;
     return;
   }
   const int64_t o = part_offsets[i];
@@ -221,13 +242,7 @@ __global__ void __launch_bounds__(kBlock) k_boxes_sub(TreeView t, const double* 
   if (WRITE && hits != nullptr) {
     const int64_t total = (i + 1 < n && subs[i + 1].query == sq.query ? prefix[i + 1] : query_offsets[q + 1] -
                            query_offsets[q]) - prefix[i];
-    if (total <= slots) {
-      for (int64_t h = 0; h < total; ++h) {
-        ii[o + h] = q;
-        jj[o + h] = hits[h * n + i];
-      }
-      return;
-    }
+    if (total <= slots) return;  // written from the record by k_boxes_replay
   }
   const Box box = ld_box(boxes, q);
   int stack[STACK];
@@ -274,6 +289,52 @@ __global__ void __launch_bounds__(kBlock) k_boxes_sub(TreeView t, const double* 
     }
   }
   if (!WRITE) counts[i] = cnt;
+}
+
+// Box rows of the subqueries whose hits all fit the record, copied from it.  One wavefront
+// per workgroup: it reads its lanes' record slots coalesced (slot-major: slot h of lane j at
+// hits[h * n + j]) into LDS, 32 slots at a time, then writes the rows of two lanes per step as
+// two contiguous runs of up to 32, instead of 64 scattered 8-byte stores per instruction.
+constexpr int kReplayRows = 32;
+
+__global__ void __launch_bounds__(64) k_boxes_replay(const ct_subquery* __restrict__ subs, int64_t n,
+                                                     const int64_t* __restrict__ query_offsets,
+                                                     const int64_t* __restrict__ prefix, const int32_t* __restrict__ hits,
+                                                     int slots, int64_t* __restrict__ ii, int64_t* __restrict__ jj) {
+  __shared__ int32_t buf[kReplayRows][64];
+  const int lane = threadIdx.x;
+  const int64_t i = (int64_t)blockIdx.x * 64 + lane;
+  int64_t q = 0, o = 0, total = 0;
+  if (i < n) {
+    q = subs[i].query;
+    o = query_offsets[q] + prefix[i];
+    total = (i + 1 < n && subs[i + 1].query == q ? prefix[i + 1] : query_offsets[q + 1] - query_offsets[q]) - prefix[i];
+    if (total > slots) total = 0;  // walked again by k_boxes_sub
+  }
+  int64_t H = total;
+  for (int off = 32; off > 0; off >>= 1) {
+    const int64_t other = __shfl_xor(H, off);
+    H = other > H ? other : H;
+  }
+  for (int64_t h0 = 0; h0 < H; h0 += kReplayRows) {
+    for (int c = 0; c < kReplayRows; ++c) {
+      const int64_t h = h0 + c;
+      buf[c][lane] = h < total ? hits[h * n + i] : 0;
+    }
+    __syncthreads();
+    // two lanes' rows per step: lanes 0-31 write lane l's, lanes 32-63 lane l + 1's
+    const int half = lane >> 5, r = lane & 31;
+    const int64_t h = h0 + r;
+    for (int l = 0; l < 64; l += 2) {
+      const int64_t tl = __shfl(total, l + half);
+      const int64_t ol = __shfl(o, l + half), ql = __shfl(q, l + half);
+      if (h < tl) {
+        jj[ol + h] = buf[r][l + half];
+        ii[ol + h] = ql;
+      }
+    }
+    __syncthreads();
+  }
 }
 
 // Segment subquery j = (q, start): edge_query from `start`; count or write pass.
@@ -400,7 +461,7 @@ extern "C" {
 
 int ct_split_queries(const ct_tree* tree, int32_t kind, const double* queries, int64_t n, const int32_t* order,
                      int32_t min_cells, int32_t leaf_cells, int32_t max_parts, int64_t* parts,
-                     const int64_t* part_offsets, ct_subquery* subs, void* stream) {
+                     const int64_t* part_offsets, ct_subquery* subs, int32_t* frontier_buf, void* stream) {
   int rc = check_split_tree(tree);
   if (rc) return rc;
   CT_REQUIRE(kind >= 1 && kind <= 3 && n >= 0, "kind must be 1 (boxes), 2 (segment-face) or 3 (segment-edge)");
@@ -416,8 +477,10 @@ int ct_split_queries(const ct_tree* tree, int32_t kind, const double* queries, i
   const TreeView t = view(tree);
   hipStream_t st = as_stream(stream);
   const int k = kind == 1 ? 1 : 2;  // box or segment
-  if (kind == 2) k_split<true><<<grid_for(n), kBlock, 0, st>>>(t, k, queries, order, n, p, parts, part_offsets, subs);
-  else k_split<false><<<grid_for(n), kBlock, 0, st>>>(t, k, queries, order, n, p, parts, part_offsets, subs);
+  if (kind == 2)
+    k_split<true><<<grid_for(n), kBlock, 0, st>>>(t, k, queries, order, n, p, parts, part_offsets, subs, frontier_buf);
+  else
+    k_split<false><<<grid_for(n), kBlock, 0, st>>>(t, k, queries, order, n, p, parts, part_offsets, subs, frontier_buf);
   CT_LAUNCH_CHECK();
   return CT_OK;
 }
@@ -451,10 +514,15 @@ int ct_locate_boxes_write_sub(const ct_tree* tree, const double* boxes, const ct
   CT_REQUIRE(n_sub >= 0 && (n_sub == 0 || (subs && query_offsets && prefix)), "bad arguments");
   CT_REQUIRE(hits == nullptr || (hit_slots >= 1 && hit_slots <= 1024), "hit_slots must be in [1, 1024]");
   if (n_sub == 0) return CT_OK;
+  hipStream_t st = as_stream(stream);
+  if (hits != nullptr) {
+    k_boxes_replay<<<(unsigned)((n_sub + 63) / 64), 64, 0, st>>>(subs, n_sub, query_offsets, prefix, hits, hit_slots,
+                                                                 ii, jj);
+    CT_LAUNCH_CHECK();
+  }
   const TreeView t = view(tree);
   return with_stack(tree->depth, BoxesSubLaunch<true>{t, boxes, subs, n_sub, nullptr, query_offsets, prefix, ii, jj,
-                                                      const_cast<int32_t*>(hits), hits ? hit_slots : 0,
-                                                      as_stream(stream)});
+                                                      const_cast<int32_t*>(hits), hits ? hit_slots : 0, st});
 }
 
 int ct_locate_edges_count_sub(const ct_tree* tree, const double* segments, const ct_subquery* subs, int64_t n_sub,

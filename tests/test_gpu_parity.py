@@ -745,13 +745,21 @@ def test_split_queries_are_exact(trees, max_parts, leaf_cells, slots):
 
     import torch
 
+    from numba_celltree_amd import _native
+
     d = torch.from_numpy(boxes).cuda()
+    order = D.query_order(1, d, g._bbox)
     g.SPLIT_MIN_CELLS, g.SPLIT_MAX_PARTS, g.SPLIT_LEAF_CELLS = 0, max_parts, leaf_cells
     try:
-        _, _, m = g._split(1, d, D.query_order(1, d, g._bbox))
+        subs, part_offsets, m = g._split(1, d, order)
     finally:
         del g.SPLIT_MIN_CELLS, g.SPLIT_MAX_PARTS, g.SPLIT_LEAF_CELLS
     assert m > n
+    # without the frontier buffer the write call computes the frontiers again: same table
+    again = D.empty((m, 2), torch.int32)
+    _native.call("ct_split_queries", g.tree_ref, 1, D.ptr(d), n, D.ptr(order), 0, leaf_cells, max_parts, None,
+                 D.ptr(part_offsets), D.ptr(again), None, D.stream())
+    assert torch.equal(again, subs[:m])
 
 
 def test_split_edge_tree_is_exact():
