@@ -325,14 +325,29 @@ int ct_locate_boxes_write_sub(const ct_tree* tree, const double* boxes, const ct
                               const int64_t* query_offsets, const int64_t* prefix, const int32_t* hits,
                               int32_t hit_slots, int64_t* ii, int64_t* jj, void* stream);
 
-/* query.py:500-544 locate_edge_faces (kind 1) / locate_edge_edges (kind 0) over subqueries,
- * as the box pair above; rows in DFS order per query (ct_sort_intersections_by_edge then
- * gives the reference's order). */
+/* query.py:500-544 locate_edge_faces (kind 1) / locate_edge_edges (kind 0) over subqueries.
+ * Count pass: counts[j] per subquery and, with hits, the record of its first hits;
+ * hit_xy (optional, with hits) keeps the clipped segment of each of the first xy_slots
+ * (<= hit_slots) hits, hit_xy[4 * (slot * n_sub + j) + 0..3] = (x0, y0, x1, y1).
+ * Then sub_offsets = ct_exclusive_scan(counts) (staging rows in processing order: subquery j
+ * at sub_offsets[j], rows in DFS order) and ct_subquery_spans (per query: its row count, to
+ * scan into the output offsets, and its first staging row).  Write pass: the staging rows
+ * (query, cell, clipped segment, sort key t = (x0 - a) . (b - a)), copied from the record for
+ * subqueries whose hits all have a clipped segment in it.  ct_sort_staged_rows then writes the
+ * output rows: per query at offsets[q], ordered by t as sort_intersections_by_edge
+ * (geometry_utils.py:564-574). */
 int ct_locate_edges_count_sub(const ct_tree* tree, const double* segments, const ct_subquery* subs, int64_t n_sub,
-                              int32_t kind, int64_t* counts, int32_t* hits, int32_t hit_slots, void* stream);
+                              int32_t kind, int64_t* counts, int32_t* hits, int32_t hit_slots, double* hit_xy,
+                              int32_t xy_slots, void* stream);
+int ct_subquery_spans(const int32_t* order, int64_t n, const int64_t* part_offsets, const int64_t* sub_offsets,
+                      int64_t* per_query, int64_t* query_start, void* stream);
 int ct_locate_edges_write_sub(const ct_tree* tree, const double* segments, const ct_subquery* subs, int64_t n_sub,
-                              int32_t kind, const int64_t* query_offsets, const int64_t* prefix, const int32_t* hits,
-                              int32_t hit_slots, int64_t* ii, int64_t* jj, double* xy, void* stream);
+                              int32_t kind, const int64_t* sub_offsets, const int32_t* hits, int32_t hit_slots,
+                              const double* hit_xy, int32_t xy_slots, int32_t* rows_query, int32_t* rows_cell,
+                              double* rows_xy, double* rows_t, void* stream);
+int ct_sort_staged_rows(const int64_t* offsets, const int64_t* query_start, const int32_t* rows_query,
+                        const int32_t* rows_cell, const double* rows_xy, const double* rows_t, int64_t n_rows,
+                        int64_t* ii, int64_t* jj, double* xy, void* stream);
 
 /* geometry_utils.py:564-574 sort_intersections_by_edge on grouped output (ii ascending, the
  * rows of segment s in [offsets[s], offsets[s+1])): writes jj and xy (n_rows, 2, 2) stably

@@ -113,14 +113,14 @@ def empty(shape, dtype) -> torch.Tensor:
     return torch.empty(shape, dtype=dtype, device=device())
 
 
-def exclusive_scan(counts: torch.Tensor):
-    """Offsets (n + 1, int64, device) of ``counts`` and the total (host int, synchronizes)."""
+def exclusive_scan(counts: torch.Tensor, total: bool = True):
+    """Offsets (n + 1, int64, device) of ``counts`` and the total (host int: synchronizes;
+    None with ``total=False``)."""
     n = counts.numel()
     offsets = empty(n + 1, torch.int64)
     ws = empty(int(_native.lib().ct_scan_workspace_bytes(n)), torch.uint8)
     _native.call("ct_exclusive_scan", ptr(counts), n, ptr(offsets), ptr(ws), ws.numel(), stream())
-    total = int(offsets[n].item())
-    return offsets, total
+    return offsets, (int(offsets[n].item()) if total else None)
 
 
 def query_order(kind: int, queries: torch.Tensor, bbox, with_sorted: bool = False):

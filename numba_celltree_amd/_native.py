@@ -158,8 +158,11 @@ SIGNATURES = {
     "ct_subquery_offsets": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P]),
     "ct_locate_boxes_count_sub": (ctypes.c_int, [_TP, _P, _P, _I64, _P, _P, _I32, _P]),
     "ct_locate_boxes_write_sub": (ctypes.c_int, [_TP, _P, _P, _I64, _P, _P, _P, _I32, _P, _P, _P]),
-    "ct_locate_edges_count_sub": (ctypes.c_int, [_TP, _P, _P, _I64, _I32, _P, _P, _I32, _P]),
-    "ct_locate_edges_write_sub": (ctypes.c_int, [_TP, _P, _P, _I64, _I32, _P, _P, _P, _I32, _P, _P, _P, _P]),
+    "ct_locate_edges_count_sub": (ctypes.c_int, [_TP, _P, _P, _I64, _I32, _P, _P, _I32, _P, _I32, _P]),
+    "ct_locate_edges_write_sub": (ctypes.c_int, [_TP, _P, _P, _I64, _I32, _P, _P, _I32, _P, _I32, _P, _P, _P, _P,
+                                                 _P]),
+    "ct_subquery_spans": (ctypes.c_int, [_P, _I64, _P, _P, _P, _P, _P]),
+    "ct_sort_staged_rows": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P]),
     "ct_locate_points_ordered_noalloc": (ctypes.c_int, [_TP, _P, _I64, _F64, _P, _P]),
     "ct_locate_points_on_edge": (ctypes.c_int, [_TP, _P, _I64, _F64, _P, _P]),
     "ct_order_workspace_bytes": (_SZ, [_I64]),

@@ -368,8 +368,14 @@ class CellTree2dBase:
     EDGE_HIT_SLOTS = 256
     EDGE_HIT_BUDGET = 1 << 28
 
-    # Box queries: a box's hits are its output rows, and wide boxes have hundreds (cfg2: up to
-    # ~900), so the record is larger: up to BOX_HIT_SLOTS slots within BOX_HIT_BUDGET int32
+    # With the cells, the count pass of split segment queries keeps the clipped segments of the
+    # first EDGE_XY_SLOTS hits (32 bytes each, within EDGE_XY_BUDGET bytes), so that the write
+    # pass copies the rows of a subquery whose hits all fit instead of clipping its cells again.
+    EDGE_XY_SLOTS = 32
+    EDGE_XY_BUDGET = 1 << 31
+
+    # Box queries: a box's hits are its output rows, and wide boxes have hundreds (up to ~900
+    # for boxes of 20 cell widths on a million-cell mesh), so the record is larger: up to BOX_HIT_SLOTS slots within BOX_HIT_BUDGET int32
     # entries (4 GiB).  All four are class defaults; setting them on a tree instance changes
     # that tree only (results never depend on them).
     BOX_HIT_SLOTS = 1024
@@ -390,11 +396,28 @@ class CellTree2dBase:
                 slots //= 2
         return None, 0
 
+    @staticmethod
+    def _xy_record(n: int, slots: int, max_slots: int, budget: int):
+        """(hit_xy, xy_slots): the clipped-segment record beside a cell record of `slots` slots,
+        up to max_slots slots within budget bytes; (None, 0) below 4 slots or when the device
+        cannot hold it."""
+        xs = min(int(max_slots), int(slots), int(budget) // (32 * max(n, 1)))
+        while xs >= 4:
+            try:
+                return D.empty((xs * max(n, 1), 4), torch.float64), xs
+            except torch.cuda.OutOfMemoryError:
+                torch.cuda.empty_cache()
+                xs //= 2
+        return None, 0
+
     # Work splitting of the largest box and segment queries (ct_split_queries): a query expected
     # to meet more than SPLIT_MIN_CELLS cells is cut into at most SPLIT_MAX_PARTS subqueries by
-    # subtree, expanding subtrees of more than SPLIT_LEAF_CELLS cells; results are identical.
+    # subtree, expanding subtrees of more than SPLIT_LEAF_CELLS (boxes) or EDGE_SPLIT_LEAF_CELLS
+    # (segments) cells; results are identical.  Segments take larger leaves: fewer subqueries
+    # keep their hit records small.
     SPLIT_MIN_CELLS = 16
-    SPLIT_LEAF_CELLS = 16
+    SPLIT_LEAF_CELLS = 32
+    EDGE_SPLIT_LEAF_CELLS = 64
     SPLIT_MAX_PARTS = 16
 
     def _split(self, kind: int, queries: torch.Tensor, order):
@@ -404,8 +427,9 @@ class CellTree2dBase:
         if order is None or self.depth > _native.CT_MAX_DEPTH or int(self.SPLIT_MAX_PARTS) <= 1:
             return None
         st = D.stream()
-        args = (self.tree_ref, kind, D.ptr(queries), n, D.ptr(order), int(self.SPLIT_MIN_CELLS),
-                int(self.SPLIT_LEAF_CELLS), int(self.SPLIT_MAX_PARTS))
+        leaf = self.SPLIT_LEAF_CELLS if kind == 1 else self.EDGE_SPLIT_LEAF_CELLS
+        args = (self.tree_ref, kind, D.ptr(queries), n, D.ptr(order), int(self.SPLIT_MIN_CELLS), int(leaf),
+                int(self.SPLIT_MAX_PARTS))
         parts = D.empty(n, torch.int64)
         keep = D.empty(n * int(self.SPLIT_MAX_PARTS), torch.int32)  # frontiers, for the write call
         _native.call("ct_split_queries", *args, D.ptr(parts), None, None, D.ptr(keep), st)
@@ -442,15 +466,30 @@ class CellTree2dBase:
             subs, part_offsets, m = split
             counts = D.empty(max(m, 1), torch.int64)
             hits, slots = self._hit_record(m, self.EDGE_HIT_SLOTS, self.EDGE_HIT_BUDGET)
+            hit_xy, xy_slots = (None, 0) if hits is None else self._xy_record(m, slots, self.EDGE_XY_SLOTS,
+                                                                              self.EDGE_XY_BUDGET)
             _native.call("ct_locate_edges_count_sub", self.tree_ref, D.ptr(edge_coords), D.ptr(subs), m, kind,
-                         D.ptr(counts), D.ptr(hits), slots, st)
-            offsets, total = self._sub_offsets(order, n, part_offsets, counts)
+                         D.ptr(counts), D.ptr(hits), slots, D.ptr(hit_xy), xy_slots, st)
+            # staging rows in processing order, then the per-segment sort into the output rows
+            sub_offsets, total = D.exclusive_scan(counts[:m])
+            per_query = D.empty(n, torch.int64)
+            query_start = D.empty(n, torch.int64)
+            _native.call("ct_subquery_spans", D.ptr(order), n, D.ptr(part_offsets), D.ptr(sub_offsets),
+                         D.ptr(per_query), D.ptr(query_start), st)
+            offsets, _ = D.exclusive_scan(per_query, total=False)
+            rows_query = D.empty(total, torch.int32)
+            rows_cell = D.empty(total, torch.int32)
+            rows_xy = D.empty((total, 4), torch.float64)
+            rows_t = D.empty(total, torch.float64)
+            _native.call("ct_locate_edges_write_sub", self.tree_ref, D.ptr(edge_coords), D.ptr(subs), m, kind,
+                         D.ptr(sub_offsets), D.ptr(hits), slots, D.ptr(hit_xy), xy_slots, D.ptr(rows_query),
+                         D.ptr(rows_cell), D.ptr(rows_xy), D.ptr(rows_t), st)
             ii = D.empty(total, torch.int64)
             jj = D.empty(total, torch.int64)
             xy = D.empty((total, 2, 2), torch.float64)
-            _native.call("ct_locate_edges_write_sub", self.tree_ref, D.ptr(edge_coords), D.ptr(subs), m, kind,
-                         D.ptr(offsets), D.ptr(counts), D.ptr(hits), slots, D.ptr(ii), D.ptr(jj), D.ptr(xy), st)
-            return self._sort_edges(edge_coords, offsets, ii, jj, xy, total)
+            _native.call("ct_sort_staged_rows", D.ptr(offsets), D.ptr(query_start), D.ptr(rows_query),
+                         D.ptr(rows_cell), D.ptr(rows_xy), D.ptr(rows_t), total, D.ptr(ii), D.ptr(jj), D.ptr(xy), st)
+            return ii, jj, xy
         counts = D.empty(n, torch.int64)
         hits, slots = self._hit_record(n, self.EDGE_HIT_SLOTS, self.EDGE_HIT_BUDGET)
         _native.call(

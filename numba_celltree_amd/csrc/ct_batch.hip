@@ -136,13 +136,13 @@ __global__ void __launch_bounds__(kBatchMaxBlock) k_batch_edges(BatchArgs a, con
               reinterpret_cast<double4*>(xy)[o + h] = make_double4(cc.x, cc.y, dd.x, dd.y);
             }
           } else {
-            edge_query<true, MAXV, STACK, true>(t, pa, pb, q, o, ii, jj, xy, stack);
+            edge_query<true, MAXV, STACK, true>(t, pa, pb, q, o, FinalRows{ii, jj, xy}, stack);
           }
         }
       } else {
         HitRecord rec;
         if (hits != nullptr) rec = HitRecord{hits + q, slots, n_queries};
-        counts[q] = edge_query<false, MAXV, STACK, true>(t, pa, pb, q, 0, nullptr, nullptr, nullptr, stack, rec);
+        counts[q] = edge_query<false, MAXV, STACK, true>(t, pa, pb, q, 0, FinalRows{}, stack, rec);
       }
     }
   }
@@ -157,11 +157,11 @@ __global__ void __launch_bounds__(kBlock) k_batch_sort(const double* __restrict_
                                                        const int64_t* __restrict__ jj_in,
                                                        const double* __restrict__ xy_in, int64_t capacity,
                                                        int64_t* __restrict__ jj_out, double* __restrict__ xy_out) {
+  __shared__ double tw[kRankWindow];
   const int64_t total = offsets[n_segments];
   const int64_t rows = (total <= capacity) ? total : 0;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += stride)
-    sort_row(segs, offsets, ii, jj_in, xy_in, r, jj_out, xy_out);
+  for (int64_t r0 = (int64_t)blockIdx.x * kRankTile; r0 < rows; r0 += (int64_t)gridDim.x * kRankTile)
+    sort_tile(segs, offsets, ii, jj_in, xy_in, rows, r0, jj_out, xy_out, tw);
 }
 
 struct PointsL {

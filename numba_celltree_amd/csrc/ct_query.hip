@@ -533,12 +533,12 @@ __global__ void __launch_bounds__(kBlock) k_edges(TreeView t, const double* __re
         reinterpret_cast<double4*>(xy)[o + h] = make_double4(cc.x, cc.y, dd.x, dd.y);
       }
     } else {
-      edge_query<true, MAXV, STACK, FACES>(t, a, b, q, o, ii, jj, xy, stack);
+      edge_query<true, MAXV, STACK, FACES>(t, a, b, q, o, FinalRows{ii, jj, xy}, stack);
     }
   } else {
     HitRecord rec;
     if (hits != nullptr) rec = HitRecord{hits + i, slots, n};
-    counts[q] = edge_query<false, MAXV, STACK, FACES>(t, a, b, q, 0, nullptr, nullptr, nullptr, stack, rec);
+    counts[q] = edge_query<false, MAXV, STACK, FACES>(t, a, b, q, 0, FinalRows{}, stack, rec);
   }
 }
 
@@ -549,9 +549,8 @@ __global__ void __launch_bounds__(kBlock) k_sort_by_edge(const double* __restric
                                                          const double* __restrict__ xy_in,
                                                          int64_t n_rows, int64_t* __restrict__ jj_out,
                                                          double* __restrict__ xy_out) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n_rows) return;
-  sort_row(segs, offsets, ii, jj_in, xy_in, r, jj_out, xy_out);
+  __shared__ double tw[kRankWindow];
+  sort_tile(segs, offsets, ii, jj_in, xy_in, n_rows, (int64_t)blockIdx.x * kRankTile, jj_out, xy_out, tw);
 }
 
 // ----------------------------------------------------------------- barycentric
@@ -1206,8 +1205,8 @@ int ct_sort_intersections_by_edge(const double* segments, const int64_t* offsets
                                   int64_t* jj_out, double* xy_out, void* stream) {
   CT_REQUIRE(n_rows >= 0, "n_rows must be >= 0");
   if (n_rows == 0) return CT_OK;
-  k_sort_by_edge<<<grid_for(n_rows), kBlock, 0, as_stream(stream)>>>(segments, offsets, ii, jj, xy, n_rows,
-                                                                     jj_out, xy_out);
+  k_sort_by_edge<<<grid_for(n_rows, kRankTile), kBlock, 0, as_stream(stream)>>>(segments, offsets, ii, jj, xy,
+                                                                                n_rows, jj_out, xy_out);
   CT_LAUNCH_CHECK();
   return CT_OK;
 }

@@ -50,8 +50,8 @@ struct SplitQuery {
   int kind;
   Box box;      // kind 1
   P2 a, b;      // kind 2
-  double vx, vy, mseg;
-  const double4* nbox;
+  double vx, vy;
+  SegPrune sp;
 };
 
 template <bool FACES>
@@ -67,7 +67,7 @@ __device__ __forceinline__ SplitQuery load_split_query(const TreeView& t, int ki
     s.b = {d.z, d.w};
     s.vx = s.b.x - s.a.x;
     s.vy = s.b.y - s.a.y;
-    s.nbox = seg_prune_boxes<FACES>(t, s.a, s.b, s.vx, s.vy, &s.mseg);
+    s.sp = seg_prune<FACES>(t, s.a, s.b, s.vx, s.vy);
   }
   return s;
 }
@@ -84,29 +84,23 @@ __device__ __forceinline__ bool worth_splitting(const SplitQuery& s, const Split
 template <bool FACES>
 __device__ __forceinline__ void children(const SplitQuery& s, const NodeLd& nd, bool* left, bool* right) {
   if (s.kind == 1) box_children(nd, s.box, left, right);
-  else seg_children(nd, s.a, s.b, s.vx, s.vy, s.mseg, s.nbox, left, right);
+  else seg_children(nd, s.a, s.b, s.vx, s.vy, s.sp, left, right);
 }
 
-// Frontier entries: node index, with bit 31 set when the subtree is worth expanding (an
-// internal node holding more than leaf_cells cells).
-constexpr int kExpand = 1 << 31;
-constexpr int kNodeMask = 0x7fffffff;
-
-__device__ __forceinline__ int frontier_entry(const ct_node* __restrict__ nodes, int node, int leaf_cells) {
-  const NodeLd nd = ld_node(nodes, node);
-  return node | ((nd.child != -1 && nd.size > leaf_cells) ? kExpand : 0);
-}
+// Frontier entries: node index, with bit 30 set once the node is known to stay (a leaf, or a
+// subtree of at most leaf_cells cells), so that later rounds do not load it again.
+constexpr int kFinal = 1 << 30;
+constexpr int kNodeMask = kFinal - 1;
 
 // The frontier of query s (see the file comment); returns m (0: the query misses the tree
 // bbox) and leaves the entries in *out (one of the two LDS buffers a and b, this lane's slots
 // at stride kBlock).  Queries not worth splitting get the root alone, untested (their walk
 // tests the tree bbox itself).
 //
-// Expansion runs in rounds: every expandable entry is replaced by its viable children (in
-// visit order: right, then left) at once, so a round's node loads are independent of each
-// other; a round that would exceed max_parts entries is not taken.  The second pass of a
-// round tests an expanded entry's children again (its node record is in the cache by then)
-// rather than keeping the first pass's answer in LDS.
+// Expansion runs in rounds: every entry not yet known to stay is loaded once; an internal node
+// of more than leaf_cells cells is replaced by its viable children (in visit order: right,
+// then left), written straight into the other buffer; a round that would leave more than
+// max_parts entries is dropped (the frontier stays as it was before it).
 template <bool FACES>
 __device__ int frontier(const TreeView& t, const SplitQuery& s, const SplitParams& p, int* a, int* b, int** out) {
   a[0] = 0;
@@ -123,51 +117,51 @@ __device__ int frontier(const TreeView& t, const SplitQuery& s, const SplitParam
   int node = 0;
   while (true) {
     const NodeLd nd = ld_node(t.nodes, node);
-    if (nd.child == -1 || nd.size <= p.leaf_cells) break;
+    if (nd.child == -1 || nd.size <= p.leaf_cells) {
+      a[0] = node | kFinal;
+      return 1;
+    }
     bool left, right;
     children<FACES>(s, nd, &left, &right);
     if (left && right) break;
     if (!left && !right) return 0;
     node = nd.child + (right ? 1 : 0);
   }
-  a[0] = frontier_entry(t.nodes, node, p.leaf_cells);
+  a[0] = node;
   int* cur = a;
   int* nxt = b;
   int m = 1;
   while (true) {
-    int m2 = 0;
+    int w = 0;
     bool any = false;
     for (int k = 0; k < m; ++k) {
       const int e = cur[k * kBlock];
-      int cnt = 1;
-      if (e & kExpand) {
-        const NodeLd nd = ld_node(t.nodes, e & kNodeMask);
-        bool left, right;
-        children<FACES>(s, nd, &left, &right);
-        cnt = (left ? 1 : 0) + (right ? 1 : 0);
-        any = true;
+      int e0 = e | kFinal, e1 = -1;
+      if (!(e & kFinal)) {
+        const NodeLd nd = ld_node(t.nodes, e);
+        if (nd.child != -1 && nd.size > p.leaf_cells) {
+          bool left, right;
+          children<FACES>(s, nd, &left, &right);
+          any = true;
+          e0 = right ? nd.child + 1 : (left ? nd.child : -1);
+          e1 = (right && left) ? nd.child : -1;
+        }
       }
-      m2 += cnt;
-    }
-    if (!any || m2 > p.max_parts) break;
-    int w = 0;
-    for (int k = 0; k < m; ++k) {
-      const int e = cur[k * kBlock];
-      if (!(e & kExpand)) {
-        nxt[w++ * kBlock] = e;
-      } else {
-        const NodeLd nd = ld_node(t.nodes, e & kNodeMask);
-        bool left, right;
-        children<FACES>(s, nd, &left, &right);
-        if (right) nxt[w++ * kBlock] = frontier_entry(t.nodes, nd.child + 1, p.leaf_cells);
-        if (left) nxt[w++ * kBlock] = frontier_entry(t.nodes, nd.child, p.leaf_cells);
+      if (e0 >= 0) {
+        if (w < p.max_parts) nxt[w * kBlock] = e0;
+        ++w;
+      }
+      if (e1 >= 0) {
+        if (w < p.max_parts) nxt[w * kBlock] = e1;
+        ++w;
       }
     }
+    if (w > p.max_parts) break;  // keep cur
     int* tmp = cur;
     cur = nxt;
     nxt = tmp;
-    m = m2;
-    if (m == 0) break;
+    m = w;
+    if (!any || m == 0) break;
   }
   *out = cur;
   return m;
@@ -337,44 +331,273 @@ __global__ void __launch_bounds__(64) k_boxes_replay(const ct_subquery* __restri
   }
 }
 
-// Segment subquery j = (q, start): edge_query from `start`; count or write pass.
-template <bool WRITE, int MAXV, int STACK, bool FACES>
-__global__ void __launch_bounds__(kBlock) k_edges_sub(TreeView t, const double* __restrict__ segs,
-                                                      const ct_subquery* __restrict__ subs, int64_t n,
-                                                      int64_t* __restrict__ counts,
-                                                      const int64_t* __restrict__ query_offsets,
-                                                      const int64_t* __restrict__ prefix, int64_t* __restrict__ ii,
-                                                      int64_t* __restrict__ jj, double* __restrict__ xy,
-                                                      int32_t* __restrict__ hits, int slots) {
+// Segment subquery i = (q, start): edge_query from `start`, count pass (counts[i], and the
+// first hits in the record when there is one).
+template <int MAXV, int STACK, bool FACES>
+__global__ void __launch_bounds__(kBlock) k_edges_count(TreeView t, const double* __restrict__ segs,
+                                                        const ct_subquery* __restrict__ subs, int64_t n,
+                                                        int64_t* __restrict__ counts, int32_t* __restrict__ hits,
+                                                        int slots, double4* __restrict__ hit_xy, int xy_slots) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const ct_subquery sq = subs[i];
+  const double4 s4 = reinterpret_cast<const double4*>(segs)[sq.query];
+  CT_LANE_STACK(STACK, stack);
+  HitRecord rec;
+  if (hits != nullptr) rec = HitRecord{hits + i, slots, n, hit_xy != nullptr ? hit_xy + i : nullptr, xy_slots};
+  counts[i] = edge_query<false, MAXV, STACK, FACES>(t, {s4.x, s4.y}, {s4.z, s4.w}, sq.query, 0, FinalRows{}, stack,
+                                                    rec, sq.node);
+}
+
+// Write pass into the staging rows (processing order: subquery i's rows at sub_offsets[i] + h)
+// for the subqueries whose rows are not all in the record with their clipped segments: cells
+// recorded are clipped again, others walk again.  The rest are copied by k_edges_rows.
+template <int MAXV, int STACK, bool FACES>
+__global__ void __launch_bounds__(kBlock) k_edges_stage(TreeView t, const double* __restrict__ segs,
+                                                        const ct_subquery* __restrict__ subs, int64_t n,
+                                                        const int64_t* __restrict__ sub_offsets,
+                                                        const int32_t* __restrict__ hits, int slots, int xy_slots,
+                                                        StagedRows rows) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t o = sub_offsets[i], total = sub_offsets[i + 1] - o;
+  if (total == 0 || (hits != nullptr && total <= xy_slots)) return;
   const ct_subquery sq = subs[i];
   const int64_t q = sq.query;
   const double4 s4 = reinterpret_cast<const double4*>(segs)[q];
   const P2 a = {s4.x, s4.y}, b = {s4.z, s4.w};
-  CT_LANE_STACK(STACK, stack);
-  if (WRITE) {
-    const int64_t o = query_offsets[q] + prefix[i];
-    const int64_t total = (i + 1 < n && subs[i + 1].query == sq.query ? prefix[i + 1] : query_offsets[q + 1] -
-                           query_offsets[q]) - prefix[i];
-    if (hits != nullptr && total <= slots) {
-      for (int64_t h = 0; h < total; ++h) {
-        const int c = hits[h * n + i];
-        P2 cc, dd;
-        if (FACES) edge_face<MAXV>(t, c, a, b, &cc, &dd);
-        else edge_edge(t, c, a, b, &cc, &dd);
-        ii[o + h] = q;
-        jj[o + h] = c;
-        reinterpret_cast<double4*>(xy)[o + h] = make_double4(cc.x, cc.y, dd.x, dd.y);
-      }
-    } else {
-      edge_query<true, MAXV, STACK, FACES>(t, a, b, q, o, ii, jj, xy, stack, HitRecord{}, sq.node);
+  rows.seg = s4;
+  if (hits != nullptr && total <= slots) {
+    for (int64_t h = 0; h < total; ++h) {
+      const int c = hits[h * n + i];
+      P2 cc, dd;
+      if (FACES) edge_face<MAXV>(t, c, a, b, &cc, &dd);
+      else edge_edge(t, c, a, b, &cc, &dd);
+      rows.put(o + h, q, c, cc, dd);
     }
-  } else {
-    HitRecord rec;
-    if (hits != nullptr) rec = HitRecord{hits + i, slots, n};
-    counts[i] = edge_query<false, MAXV, STACK, FACES>(t, a, b, q, 0, nullptr, nullptr, nullptr, stack, rec, sq.node);
+    return;
   }
+  CT_LANE_STACK(STACK, stack);
+  edge_query<true, MAXV, STACK, FACES>(t, a, b, q, o, rows, stack, HitRecord{}, sq.node);
+}
+
+// The staging rows of the subqueries whose hits all sit in the record with their clipped
+// segments (sub_offsets[i + 1] - sub_offsets[i] <= xy_slots), copied by whole wavefronts: the
+// 64 subqueries of a wavefront own one run of consecutive staging rows, written 64 rows per
+// step (row g of the run by lane g mod 64, its owner found by a binary search over the
+// wavefront's exclusive prefix), instead of each lane storing its own rows one by one.
+__global__ void __launch_bounds__(kBlock) k_edges_rows(const double* __restrict__ segs,
+                                                       const ct_subquery* __restrict__ subs, int64_t n,
+                                                       const int64_t* __restrict__ sub_offsets,
+                                                       const int32_t* __restrict__ hits,
+                                                       const double4* __restrict__ hit_xy, int xy_slots,
+                                                       StagedRows rows) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t wave0 = i - lane;
+  int m = 0, q = 0;
+  long long o = 0;
+  double4 s4 = make_double4(0.0, 0.0, 0.0, 0.0);
+  if (i < n) {
+    o = sub_offsets[i];
+    const int64_t total = sub_offsets[i + 1] - o;
+    if (total <= xy_slots) m = (int)total;
+    if (m > 0) {
+      q = subs[i].query;
+      s4 = reinterpret_cast<const double4*>(segs)[q];
+    }
+  }
+  int incl = m;
+  for (int d = 1; d < 64; d <<= 1) {
+    const int v = __shfl_up(incl, d);
+    if (lane >= d) incl += v;
+  }
+  const int excl = incl - m;
+  const int total = __shfl(incl, 63);
+  for (int base = 0; base < total; base += 64) {
+    const int g = base + lane;
+    int own = 0;
+    for (int step = 32; step > 0; step >>= 1) {
+      const int e = __shfl(excl, own + step);
+      if (e <= g) own += step;
+    }
+    const int h = g - __shfl(excl, own);
+    const long long r = __shfl(o, own) + h;
+    const int oq = __shfl(q, own);
+    const double4 seg = make_double4(__shfl(s4.x, own), __shfl(s4.y, own), __shfl(s4.z, own), __shfl(s4.w, own));
+    if (g < total) {
+      const int64_t src = (int64_t)h * n + wave0 + own;
+      const double4 xy = hit_xy[src];
+      rows.query[r] = oq;
+      rows.cell[r] = hits[src];
+      rows.xy[r] = xy;
+      rows.t[r] = row_t(seg, make_double2(xy.x, xy.y));
+    }
+  }
+}
+
+// Per processing position i (query q = order[i], subqueries [part_offsets[i],
+// part_offsets[i + 1])): the query's row count and its first staging row.
+__global__ void __launch_bounds__(kBlock) k_query_spans(const int32_t* __restrict__ order, int64_t n,
+                                                        const int64_t* __restrict__ part_offsets,
+                                                        const int64_t* __restrict__ sub_offsets,
+                                                        int64_t* __restrict__ per_query,
+                                                        int64_t* __restrict__ query_start) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t q = query_index(order, i);
+  const int64_t lo = sub_offsets[part_offsets[i]], hi = sub_offsets[part_offsets[i + 1]];
+  per_query[q] = hi - lo;
+  query_start[q] = lo;
+}
+
+// geometry_utils.py:564-574 on the staging rows: each row's rank by t within its query's
+// staging run [query_start[q], + rows of q) (ranked as sort_tile does, from an LDS window of
+// t), written to output row offsets[q] + rank.  A tile's rows of one query land in one output
+// block, so output lines are written whole.
+__global__ void __launch_bounds__(kBlock) k_sort_staged(const int64_t* __restrict__ offsets,
+                                                        const int64_t* __restrict__ query_start,
+                                                        const int32_t* __restrict__ rq,
+                                                        const int32_t* __restrict__ rcell,
+                                                        const double4* __restrict__ rxy,
+                                                        const double* __restrict__ rt, int64_t n_rows,
+                                                        int64_t* __restrict__ ii, int64_t* __restrict__ jj,
+                                                        double4* __restrict__ xy) {
+  __shared__ double tw[kRankWindow];
+  const int64_t r0 = (int64_t)blockIdx.x * kRankTile;
+  const int64_t w0 = r0 > kRankHalo ? r0 - kRankHalo : 0;
+  const int64_t w1 = r0 + kRankTile + kRankHalo < n_rows ? r0 + kRankTile + kRankHalo : n_rows;
+  for (int64_t r = w0 + threadIdx.x; r < w1; r += blockDim.x) tw[r - w0] = rt[r];
+  __syncthreads();
+  for (int k = threadIdx.x; k < kRankTile; k += blockDim.x) {
+    const int64_t r = r0 + k;
+    if (r >= n_rows) break;
+    const int64_t q = rq[r];
+    const int64_t out0 = offsets[q];
+    const int64_t lo = query_start[q], hi = lo + (offsets[q + 1] - out0);
+    const double t = tw[r - w0];
+    int64_t rank = 0;
+    if (lo >= w0 && hi <= w1) {
+      for (int64_t m = lo; m < r; ++m) rank += t_less(t, tw[m - w0]) ? 0 : 1;
+      for (int64_t m = r + 1; m < hi; ++m) rank += t_less(tw[m - w0], t) ? 1 : 0;
+    } else {
+      for (int64_t m = lo; m < r; ++m) rank += t_less(t, rt[m]) ? 0 : 1;
+      for (int64_t m = r + 1; m < hi; ++m) rank += t_less(rt[m], t) ? 1 : 0;
+    }
+    ii[out0 + rank] = q;
+    jj[out0 + rank] = rcell[r];
+    xy[out0 + rank] = rxy[r];
+  }
+}
+
+// Segment-face count pass with a record, first half: the walk, counting and recording the
+// candidates (cells whose bbox Cohen-Sutherland accepts, by leaf position) of subquery i.
+template <int MAXV, int STACK>
+__global__ void __launch_bounds__(kBlock) k_edges_cand(TreeView t, const double* __restrict__ segs,
+                                                       const ct_subquery* __restrict__ subs, int64_t n,
+                                                       int64_t* __restrict__ counts, int32_t* __restrict__ hits,
+                                                       int slots) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const ct_subquery sq = subs[i];
+  const double4 s4 = reinterpret_cast<const double4*>(segs)[sq.query];
+  CT_LANE_STACK(STACK, stack);
+  counts[i] = edge_query<false, MAXV, STACK, true, true>(t, {s4.x, s4.y}, {s4.z, s4.w}, sq.query, 0, FinalRows{},
+                                                         stack, HitRecord{hits + i, slots, n}, sq.node);
+}
+
+// Segment-face count pass, second half: subquery i's candidate leaf positions (the first
+// counts[i] slots of the record) are clipped in order; the hits are compacted in place into
+// cell indices (and clipped segments in the first xy_slots slots), counts[i] becomes the hit
+// count.  A subquery with more candidates than slots was recorded in part and is left as it
+// is (counts[i] > slots) for k_edges_overflow.
+//
+// Each wavefront clips its 64 subqueries' candidates together, 64 at a time: candidate g of
+// the wavefront's concatenated lists (lane by lane) goes to lane g mod 64, so that the clips
+// and their dependent loads run side by side instead of one list per lane.  A hit's slot is
+// its owner's hits so far plus the hits among the owner's earlier candidates of the same
+// batch (a ballot).  The lists are compacted in place: a hit's slot is at most its own
+// candidate's slot, and every candidate of a batch is read before any hit of it is written.
+template <int MAXV>
+__global__ void __launch_bounds__(kBlock) k_edges_clip(TreeView t, const double* __restrict__ segs,
+                                                       const ct_subquery* __restrict__ subs, int64_t n,
+                                                       int64_t* __restrict__ counts, int32_t* __restrict__ hits,
+                                                       int slots, double4* __restrict__ hit_xy, int xy_slots) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t wave0 = i - lane;
+  int m = 0;
+  bool mine = false;  // this lane's counts[i] is finished here
+  double4 s4 = make_double4(0.0, 0.0, 0.0, 0.0);
+  if (i < n) {
+    const int64_t mm = counts[i];
+    mine = mm <= slots;
+    if (mine) m = (int)mm;
+    if (m > 0) s4 = reinterpret_cast<const double4*>(segs)[subs[i].query];
+  }
+  int incl = m;  // inclusive prefix of m over the wavefront
+  for (int d = 1; d < 64; d <<= 1) {
+    const int v = __shfl_up(incl, d);
+    if (lane >= d) incl += v;
+  }
+  const int excl = incl - m;
+  const int total = __shfl(incl, 63);
+  int found = 0;  // hits of this lane's subquery so far
+  for (int base = 0; base < total; base += 64) {
+    const int g = base + lane;
+    // owner o: the last lane with excl[o] <= g (lanes without candidates share the next
+    // lane's excl and come before it)
+    int o = 0;
+    for (int step = 32; step > 0; step >>= 1) {
+      const int e = __shfl(excl, o + step);
+      if (e <= g) o += step;
+    }
+    const int oexcl = __shfl(excl, o);
+    const int ofound = __shfl(found, o);
+    const P2 a = {__shfl(s4.x, o), __shfl(s4.y, o)};
+    const P2 b = {__shfl(s4.z, o), __shfl(s4.w, o)};
+    const int64_t oi = wave0 + o;
+    bool hit = false;
+    int c = 0;
+    P2 cc, dd;
+    if (g < total) {
+      const int k = hits[(int64_t)(g - oexcl) * n + oi];
+      c = t.bb_indices[k];
+      hit = edge_face_clip_leaf<MAXV>(t, k, c, leaf_box(t, k, c), a, b, &cc, &dd);
+    }
+    const uint64_t mask = __ballot(hit);
+    const int first = oexcl > base ? oexcl - base : 0;  // batch lane of the owner's first candidate
+    const int local = __popcll(mask & ((1ull << lane) - 1)) - __popcll(mask & ((1ull << first) - 1));
+    if (hit) {
+      const int slot = ofound + local;
+      hits[(int64_t)slot * n + oi] = c;
+      if (slot < xy_slots) hit_xy[(int64_t)slot * n + oi] = make_double4(cc.x, cc.y, dd.x, dd.y);
+    }
+    const int lo = excl > base ? excl - base : 0;
+    const int hi = excl + m - base < 64 ? excl + m - base : 64;
+    if (hi > lo) {
+      const uint64_t upto = hi == 64 ? ~0ull : ((1ull << hi) - 1);
+      found += __popcll(mask & upto & ~((1ull << lo) - 1));
+    }
+  }
+  if (mine) counts[i] = found;
+}
+
+// The subqueries k_edges_clip left (more candidates than slots) walk again with the whole test,
+// recording their first hits.
+template <int MAXV, int STACK>
+__global__ void __launch_bounds__(kBlock) k_edges_overflow(TreeView t, const double* __restrict__ segs,
+                                                           const ct_subquery* __restrict__ subs, int64_t n,
+                                                           int64_t* __restrict__ counts, int32_t* __restrict__ hits,
+                                                           int slots, double4* __restrict__ hit_xy, int xy_slots) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || counts[i] <= slots) return;
+  const ct_subquery sq = subs[i];
+  const double4 s4 = reinterpret_cast<const double4*>(segs)[sq.query];
+  CT_LANE_STACK(STACK, stack);
+  const HitRecord rec{hits + i, slots, n, hit_xy != nullptr ? hit_xy + i : nullptr, xy_slots};
+  counts[i] = edge_query<false, MAXV, STACK, true>(t, {s4.x, s4.y}, {s4.z, s4.w}, sq.query, 0, FinalRows{}, stack,
+                                                   rec, sq.node);
 }
 
 template <bool WRITE>
@@ -399,50 +622,72 @@ struct BoxesSubLaunch {
   }
 };
 
+// Count pass (WRITE false): candidates then their clip for segment-face queries with a
+// record, else the single walk.  Write pass: the staging rows (k_edges_rows for subqueries
+// wholly in the record, k_edges_stage for the others).
 template <bool WRITE, int MAXV, bool FACES>
 struct EdgesSubLaunch {
   const TreeView& t;
   const double* segs;
   const ct_subquery* subs;
   int64_t n;
-  int64_t* counts;
-  const int64_t* qoff;
-  const int64_t* prefix;
-  int64_t* ii;
-  int64_t* jj;
-  double* xy;
+  int64_t* counts;             // count pass
+  const int64_t* sub_offsets;  // write pass
   int32_t* hits;
   int slots;
+  double4* hit_xy;
+  int xy_slots;
+  StagedRows rows;  // write pass
   hipStream_t st;
   template <int S>
   int run() {
-    k_edges_sub<WRITE, MAXV, S, FACES><<<grid_for(n), kBlock, 0, st>>>(t, segs, subs, n, counts, qoff, prefix, ii, jj,
-                                                                        xy, hits, slots);
-    CT_LAUNCH_CHECK();
-    return CT_OK;
+    const unsigned g = grid_for(n);
+    if constexpr (WRITE) {
+      if (hits != nullptr && hit_xy != nullptr) {
+        k_edges_rows<<<g, kBlock, 0, st>>>(segs, subs, n, sub_offsets, hits, hit_xy, xy_slots, rows);
+        CT_LAUNCH_CHECK();
+      }
+      k_edges_stage<MAXV, S, FACES><<<g, kBlock, 0, st>>>(t, segs, subs, n, sub_offsets, hits, slots,
+                                                           hit_xy != nullptr ? xy_slots : 0, rows);
+      CT_LAUNCH_CHECK();
+      return CT_OK;
+    } else {
+      if (FACES && hits != nullptr) {
+        k_edges_cand<MAXV, S><<<g, kBlock, 0, st>>>(t, segs, subs, n, counts, hits, slots);
+        CT_LAUNCH_CHECK();
+        k_edges_clip<MAXV><<<g, kBlock, 0, st>>>(t, segs, subs, n, counts, hits, slots, hit_xy, xy_slots);
+        CT_LAUNCH_CHECK();
+        k_edges_overflow<MAXV, S><<<g, kBlock, 0, st>>>(t, segs, subs, n, counts, hits, slots, hit_xy, xy_slots);
+        CT_LAUNCH_CHECK();
+        return CT_OK;
+      }
+      k_edges_count<MAXV, S, FACES><<<g, kBlock, 0, st>>>(t, segs, subs, n, counts, hits, slots, hit_xy, xy_slots);
+      CT_LAUNCH_CHECK();
+      return CT_OK;
+    }
   }
 };
 
 template <bool WRITE>
 int launch_edges_sub(const ct_tree* tree, const double* segs, const ct_subquery* subs, int64_t n, int kind,
-                     int64_t* counts, const int64_t* qoff, const int64_t* prefix, int64_t* ii, int64_t* jj,
-                     double* xy, int32_t* hits, int slots, hipStream_t st) {
+                     int64_t* counts, const int64_t* sub_offsets, int32_t* hits, int slots, double4* hit_xy,
+                     int xy_slots, StagedRows rows, hipStream_t st) {
   const TreeView t = view(tree);
   const int d = tree->depth, nm = tree->n_max_vert;
   if (kind == 0)
-    return with_stack(d, EdgesSubLaunch<WRITE, 4, false>{t, segs, subs, n, counts, qoff, prefix, ii, jj, xy, hits,
-                                                         slots, st});
+    return with_stack(d, EdgesSubLaunch<WRITE, 4, false>{t, segs, subs, n, counts, sub_offsets, hits, slots, hit_xy,
+                                                         xy_slots, rows, st});
   if (nm <= 4)
-    return with_stack(d, EdgesSubLaunch<WRITE, 4, true>{t, segs, subs, n, counts, qoff, prefix, ii, jj, xy, hits,
-                                                        slots, st});
+    return with_stack(d, EdgesSubLaunch<WRITE, 4, true>{t, segs, subs, n, counts, sub_offsets, hits, slots, hit_xy,
+                                                        xy_slots, rows, st});
   if (nm <= 8)
-    return with_stack(d, EdgesSubLaunch<WRITE, 8, true>{t, segs, subs, n, counts, qoff, prefix, ii, jj, xy, hits,
-                                                        slots, st});
+    return with_stack(d, EdgesSubLaunch<WRITE, 8, true>{t, segs, subs, n, counts, sub_offsets, hits, slots, hit_xy,
+                                                        xy_slots, rows, st});
   if (nm <= 16)
-    return with_stack(d, EdgesSubLaunch<WRITE, 16, true>{t, segs, subs, n, counts, qoff, prefix, ii, jj, xy, hits,
-                                                         slots, st});
-  return with_stack(d, EdgesSubLaunch<WRITE, 32, true>{t, segs, subs, n, counts, qoff, prefix, ii, jj, xy, hits,
-                                                       slots, st});
+    return with_stack(d, EdgesSubLaunch<WRITE, 16, true>{t, segs, subs, n, counts, sub_offsets, hits, slots, hit_xy,
+                                                         xy_slots, rows, st});
+  return with_stack(d, EdgesSubLaunch<WRITE, 32, true>{t, segs, subs, n, counts, sub_offsets, hits, slots, hit_xy,
+                                                       xy_slots, rows, st});
 }
 
 int check_split_tree(const ct_tree* tree) {
@@ -526,29 +771,62 @@ int ct_locate_boxes_write_sub(const ct_tree* tree, const double* boxes, const ct
 }
 
 int ct_locate_edges_count_sub(const ct_tree* tree, const double* segments, const ct_subquery* subs, int64_t n_sub,
-                              int32_t kind, int64_t* counts, int32_t* hits, int32_t hit_slots, void* stream) {
+                              int32_t kind, int64_t* counts, int32_t* hits, int32_t hit_slots, double* hit_xy,
+                              int32_t xy_slots, void* stream) {
   int rc = check_split_tree(tree);
   if (rc) return rc;
   CT_REQUIRE(n_sub >= 0 && counts && (n_sub == 0 || subs) && (kind == 0 || kind == 1), "bad arguments");
   CT_REQUIRE(kind == 1 ? tree->n_max_vert >= 3 : tree->n_max_vert == 2, "kind does not match the tree");
   CT_REQUIRE(hits == nullptr || (hit_slots >= 1 && hit_slots <= 1024), "hit_slots must be in [1, 1024]");
+  CT_REQUIRE(hit_xy == nullptr || (hits != nullptr && xy_slots >= 1 && xy_slots <= hit_slots),
+             "hit_xy needs hits and xy_slots in [1, hit_slots]");
   if (n_sub == 0) return CT_OK;
-  return launch_edges_sub<false>(tree, segments, subs, n_sub, kind, counts, nullptr, nullptr, nullptr, nullptr,
-                                 nullptr, hits, hits ? hit_slots : 0, as_stream(stream));
+  return launch_edges_sub<false>(tree, segments, subs, n_sub, kind, counts, nullptr, hits, hits ? hit_slots : 0,
+                                 reinterpret_cast<double4*>(hit_xy), hit_xy ? xy_slots : 0, StagedRows{},
+                                 as_stream(stream));
+}
+
+int ct_subquery_spans(const int32_t* order, int64_t n, const int64_t* part_offsets, const int64_t* sub_offsets,
+                      int64_t* per_query, int64_t* query_start, void* stream) {
+  CT_REQUIRE(n >= 0 && (n == 0 || (part_offsets && sub_offsets && per_query && query_start)), "bad arguments");
+  if (n == 0) return CT_OK;
+  k_query_spans<<<grid_for(n), kBlock, 0, as_stream(stream)>>>(order, n, part_offsets, sub_offsets, per_query,
+                                                               query_start);
+  CT_LAUNCH_CHECK();
+  return CT_OK;
 }
 
 int ct_locate_edges_write_sub(const ct_tree* tree, const double* segments, const ct_subquery* subs, int64_t n_sub,
-                              int32_t kind, const int64_t* query_offsets, const int64_t* prefix, const int32_t* hits,
-                              int32_t hit_slots, int64_t* ii, int64_t* jj, double* xy, void* stream) {
+                              int32_t kind, const int64_t* sub_offsets, const int32_t* hits, int32_t hit_slots,
+                              const double* hit_xy, int32_t xy_slots, int32_t* rows_query, int32_t* rows_cell,
+                              double* rows_xy, double* rows_t, void* stream) {
   int rc = check_split_tree(tree);
   if (rc) return rc;
-  CT_REQUIRE(n_sub >= 0 && (n_sub == 0 || (subs && query_offsets && prefix)) && (kind == 0 || kind == 1),
-             "bad arguments");
+  CT_REQUIRE(n_sub >= 0 && (n_sub == 0 || (subs && sub_offsets)) && (kind == 0 || kind == 1), "bad arguments");
   CT_REQUIRE(kind == 1 ? tree->n_max_vert >= 3 : tree->n_max_vert == 2, "kind does not match the tree");
   CT_REQUIRE(hits == nullptr || (hit_slots >= 1 && hit_slots <= 1024), "hit_slots must be in [1, 1024]");
+  CT_REQUIRE(hit_xy == nullptr || (hits != nullptr && xy_slots >= 1 && xy_slots <= hit_slots),
+             "hit_xy needs hits and xy_slots in [1, hit_slots]");
+  CT_REQUIRE(n_sub == 0 || (rows_query && rows_cell && rows_xy && rows_t), "null staging rows");
   if (n_sub == 0) return CT_OK;
-  return launch_edges_sub<true>(tree, segments, subs, n_sub, kind, nullptr, query_offsets, prefix, ii, jj, xy,
-                                const_cast<int32_t*>(hits), hits ? hit_slots : 0, as_stream(stream));
+  const StagedRows rows{rows_query, rows_cell, reinterpret_cast<double4*>(rows_xy), rows_t, {}};
+  return launch_edges_sub<true>(tree, segments, subs, n_sub, kind, nullptr, sub_offsets, const_cast<int32_t*>(hits),
+                                hits ? hit_slots : 0, reinterpret_cast<double4*>(const_cast<double*>(hit_xy)),
+                                hit_xy ? xy_slots : 0, rows, as_stream(stream));
+}
+
+int ct_sort_staged_rows(const int64_t* offsets, const int64_t* query_start, const int32_t* rows_query,
+                        const int32_t* rows_cell, const double* rows_xy, const double* rows_t, int64_t n_rows,
+                        int64_t* ii, int64_t* jj, double* xy, void* stream) {
+  CT_REQUIRE(n_rows >= 0, "n_rows must be >= 0");
+  if (n_rows == 0) return CT_OK;
+  CT_REQUIRE(offsets && query_start && rows_query && rows_cell && rows_xy && rows_t && ii && jj && xy,
+             "bad arguments");
+  k_sort_staged<<<grid_for(n_rows, kRankTile), kBlock, 0, as_stream(stream)>>>(
+      offsets, query_start, rows_query, rows_cell, reinterpret_cast<const double4*>(rows_xy), rows_t, n_rows, ii, jj,
+      reinterpret_cast<double4*>(xy));
+  CT_LAUNCH_CHECK();
+  return CT_OK;
 }
 
 }  // extern "C"

@@ -268,14 +268,40 @@ __device__ __forceinline__ int locate_one(const TreeView& t, P2 p, double tol, L
 // query.py:309-328 compute_edge_face_intersect: Cohen-Sutherland on the cell bbox, then
 // Cyrus-Beck on the cell with tol = max(1e-15, 1e-12 * max(w, h)).
 // `box` is cell c's bbox.
+// The second half of edge_face_box: the Cyrus-Beck clip of a cell whose bbox Cohen-Sutherland
+// accepted.
 template <int MAXV>
-__device__ __forceinline__ bool edge_face_box(const TreeView& t, int c, const Box& box, P2 a, P2 b,
-                                              P2* cc, P2* dd) {
-  if (!cohen_sutherland(a, b, box, cc, dd)) return false;
+__device__ __forceinline__ bool edge_face_clip(const TreeView& t, int c, const Box& box, P2 a, P2 b, P2* cc,
+                                               P2* dd) {
   P2 poly[MAXV];
   const int len = load_poly<MAXV>(t.elements, t.nmax, c, t.vertices, poly);
   const double tol = pmax(kMinTolerance, kToleranceFactor * pmax(box.xmax - box.xmin, box.ymax - box.ymin));
   return cyrus_beck<MAXV>(a, b, poly, len, tol, cc, dd);
+}
+
+// edge_face_clip for the cell at leaf position k: its vertices from the packed leaf rows (a
+// read that does not wait for c; slots past the polygon's length are never read), only the
+// length from its face row.
+template <int MAXV>
+__device__ __forceinline__ bool edge_face_clip_leaf(const TreeView& t, int k, int c, const Box& box, P2 a, P2 b,
+                                                    P2* cc, P2* dd) {
+  if (t.leaf == nullptr) return edge_face_clip<MAXV>(t, c, box, a, b, cc, dd);
+  P2 poly[MAXV];
+  load_leaf_poly<MAXV, false>(t.leaf, t.nmax, k, poly);
+  const int32_t* row = t.elements + (int64_t)c * t.nmax;
+  int len = t.nmax < MAXV ? t.nmax : MAXV;
+#pragma unroll
+  for (int i = MAXV - 1; i >= 3; --i)
+    if (i < t.nmax && row[i] == kFill) len = i;
+  const double tol = pmax(kMinTolerance, kToleranceFactor * pmax(box.xmax - box.xmin, box.ymax - box.ymin));
+  return cyrus_beck<MAXV>(a, b, poly, len, tol, cc, dd);
+}
+
+template <int MAXV>
+__device__ __forceinline__ bool edge_face_box(const TreeView& t, int c, const Box& box, P2 a, P2 b,
+                                              P2* cc, P2* dd) {
+  if (!cohen_sutherland(a, b, box, cc, dd)) return false;
+  return edge_face_clip<MAXV>(t, c, box, a, b, cc, dd);
 }
 
 template <int MAXV>
@@ -299,6 +325,14 @@ __device__ __forceinline__ bool edge_edge(const TreeView& t, int c, P2 a, P2 b, 
   return r;
 }
 
+// Per-segment setup of the node tests below.
+struct SegPrune {
+  const double4* nbox;  // node-box rows (node j's children at child + 1, child + 2), or null
+  double mseg;          // the segment's largest coordinate magnitude
+  double ix, iy;        // 1 / (b - a) per axis (used only where the component is not 0)
+  bool finite_v;        // b - a is finite
+};
+
 // Subtree pruning for segment-face queries (no reference counterpart).  A face is a hit only
 // if Cohen-Sutherland accepts the segment against the face's bbox, and every face bbox of a
 // subtree lies inside the subtree's tight box.  Cohen-Sutherland's clipped points stay
@@ -306,16 +340,18 @@ __device__ __forceinline__ bool edge_edge(const TreeView& t, int c, P2 a, P2 b, 
 // the segment misses the subtree box grown by m = 1e-9 * (largest coordinate magnitude),
 // no face below can be accepted: skipping the subtree changes no result, only the number
 // of faces tested (O(length) instead of O(length^2) for long segments).  Tested by
-// Liang-Barsky slab clipping of t in [0, 1]; a NaN anywhere makes the test accept.
-__device__ __forceinline__ bool seg_meets_box(P2 a, double dx, double dy, double mseg, double4 nb) {
+// Liang-Barsky slab clipping of t in [0, 1], with t from the segment's reciprocal
+// components (a relative error of an ulp or two in t moves the tested point by a few ulps of
+// the coordinates, far inside m); a NaN anywhere makes the test accept.
+__device__ __forceinline__ bool seg_meets_box(P2 a, double dx, double dy, const SegPrune& sp, double4 nb) {
   const double mb = pmax(pmax(fabs(nb.x), fabs(nb.y)), pmax(fabs(nb.z), fabs(nb.w)));
-  const double m = 1e-9 * pmax(mseg, mb);
+  const double m = 1e-9 * pmax(sp.mseg, mb);
   const double xlo = nb.x - m, xhi = nb.y + m, ylo = nb.z - m, yhi = nb.w + m;
   double t0 = 0.0, t1 = 1.0;
   if (dx == 0.0) {
     if (a.x < xlo || a.x > xhi) return false;
   } else {
-    double ta = (xlo - a.x) / dx, tb = (xhi - a.x) / dx;
+    double ta = (xlo - a.x) * sp.ix, tb = (xhi - a.x) * sp.ix;
     if (dx < 0.0) {
       const double s = ta;
       ta = tb;
@@ -327,7 +363,7 @@ __device__ __forceinline__ bool seg_meets_box(P2 a, double dx, double dy, double
   if (dy == 0.0) {
     if (a.y < ylo || a.y > yhi) return false;
   } else {
-    double ta = (ylo - a.y) / dy, tb = (yhi - a.y) / dy;
+    double ta = (ylo - a.y) * sp.iy, tb = (yhi - a.y) * sp.iy;
     if (dy < 0.0) {
       const double s = ta;
       ta = tb;
@@ -340,10 +376,13 @@ __device__ __forceinline__ bool seg_meets_box(P2 a, double dx, double dy, double
 }
 
 // Which children of internal node nd segment (a, b) (v = b - a) may reach: query.py:407-441's
-// slab test on the split dimension (the t-tests always hold once the slab test passed, but
-// are kept), then, when nbox is set (tight node boxes, the segment finite), seg_meets_box.
-__device__ __forceinline__ void seg_children(const NodeLd& nd, P2 a, P2 b, double vx, double vy, double mseg,
-                                             const double4* __restrict__ nbox, bool* left_out, bool* right_out) {
+// slab test on the split dimension, then its t-tests, then, when sp.nbox is set (tight node
+// boxes, the segment finite), seg_meets_box.  The t-tests divide a distance of the tested sign
+// by the segment's component along the split axis: for a finite, nonzero component the
+// quotient always has the sign (or the bound) they test, so they can only fail when both are
+// infinite, and are evaluated only for segments whose components are not finite.
+__device__ __forceinline__ void seg_children(const NodeLd& nd, P2 a, P2 b, double vx, double vy, const SegPrune& sp,
+                                             bool* left_out, bool* right_out) {
   const double dx = nd.dim ? vy : vx;
   const double an = nd.dim ? a.y : a.x;
   const double bn = nd.dim ? b.y : b.x;
@@ -357,39 +396,84 @@ __device__ __forceinline__ void seg_children(const NodeLd& nd, P2 a, P2 b, doubl
   }
   bool left = dl >= 0.0;
   bool right = dr <= 0.0;
-  if (dx > 0.0) {
-    if (left) left = (dl / dx) >= 0.0;
-    if (right) right = (dr / dx) <= 1.0;
-  } else if (dx < 0.0) {
-    if (left) left = (1.0 - (dl / dx)) >= 0.0;
-    if (right) right = (1.0 - (dr / dx)) <= 1.0;
+  if (!sp.finite_v) {
+    if (dx > 0.0) {
+      if (left) left = (dl / dx) >= 0.0;
+      if (right) right = (dr / dx) <= 1.0;
+    } else if (dx < 0.0) {
+      if (left) left = (1.0 - (dl / dx)) >= 0.0;
+      if (right) right = (1.0 - (dr / dx)) <= 1.0;
+    }
   }
-  if (nbox != nullptr && (left || right)) {
+  if (sp.nbox != nullptr && (left || right)) {
     // the two children's boxes are rows child + 1, child + 2: one aligned 64-byte pair
-    if (left) left = seg_meets_box(a, vx, vy, mseg, nbox[nd.child]);
-    if (right) right = seg_meets_box(a, vx, vy, mseg, nbox[nd.child + 1]);
+    if (left) left = seg_meets_box(a, vx, vy, sp, sp.nbox[nd.child]);
+    if (right) right = seg_meets_box(a, vx, vy, sp, sp.nbox[nd.child + 1]);
   }
   *left_out = left;
   *right_out = right;
 }
 
-// Per-segment setup of the subtree pruning: the node-box rows (or null when the tree has none,
-// the query is a segment-edge query, or the segment is not finite) and the segment's largest
-// coordinate magnitude.
+// The node-test setup of segment (a, b): node-box rows only for segment-face queries on a tree
+// that has them and a finite segment.
 template <bool FACES>
-__device__ __forceinline__ const double4* seg_prune_boxes(const TreeView& t, P2 a, P2 b, double vx, double vy,
-                                                          double* mseg) {
-  *mseg = pmax(pmax(fabs(a.x), fabs(a.y)), pmax(fabs(b.x), fabs(b.y)));
-  const bool prune = FACES && t.nbox != nullptr && isfinite(*mseg) && isfinite(vx) && isfinite(vy);
-  return prune ? reinterpret_cast<const double4*>(t.nbox) + 1 : nullptr;
+__device__ __forceinline__ SegPrune seg_prune(const TreeView& t, P2 a, P2 b, double vx, double vy) {
+  SegPrune sp;
+  sp.mseg = pmax(pmax(fabs(a.x), fabs(a.y)), pmax(fabs(b.x), fabs(b.y)));
+  sp.finite_v = isfinite(vx) && isfinite(vy);
+  const bool prune = FACES && t.nbox != nullptr && isfinite(sp.mseg) && sp.finite_v;
+  sp.nbox = prune ? reinterpret_cast<const double4*>(t.nbox) + 1 : nullptr;
+  sp.ix = vx != 0.0 ? 1.0 / vx : 0.0;
+  sp.iy = vy != 0.0 ? 1.0 / vy : 0.0;
+  return sp;
 }
 
 // The first `slots` hit cells of one query, kept by a count pass so that the write pass can
-// replay them instead of walking the tree again: hit c at cells[c * stride].
+// replay them instead of walking the tree again: hit h at cells[h * stride].  Segment queries
+// may also keep the first `xy_slots` (<= slots) clipped segments, hit h at xy[h * stride] as
+// (x0, y0, x1, y1), so that the write pass copies them instead of clipping again.
 struct HitRecord {
   int32_t* cells = nullptr;
   int slots = 0;
   int64_t stride = 0;
+  double4* xy = nullptr;
+  int xy_slots = 0;
+};
+
+// geometry_utils.py:564-574's sort key of a row of segment s4 = (a, b) whose clipped start
+// point is c: t = (c - a) . (b - a), x-term + y-term as the reference's np.sum over the last
+// axis.
+__device__ __forceinline__ double row_t(double4 s4, double2 c) {
+  const double abx = s4.z - s4.x, aby = s4.w - s4.y;
+  return (c.x - s4.x) * abx + (c.y - s4.y) * aby;
+}
+
+// Where a writing walk puts its rows: the output arrays (row r = (q, cell, clipped segment)) ...
+struct FinalRows {
+  int64_t* ii;
+  int64_t* jj;
+  double* xy;
+  __device__ __forceinline__ void put(int64_t r, int64_t q, int c, P2 cc, P2 dd) const {
+    ii[r] = q;
+    jj[r] = c;
+    reinterpret_cast<double4*>(xy)[r] = make_double4(cc.x, cc.y, dd.x, dd.y);
+  }
+};
+
+// ... or staging rows in processing order (ct_split.hip), with each row's sort key t for
+// segment `seg`.
+struct StagedRows {
+  int32_t* query;
+  int32_t* cell;
+  double4* xy;
+  double* t;
+  double4 seg;
+  __device__ __forceinline__ void put(int64_t r, int64_t q, int c, P2 cc, P2 dd) const {
+    query[r] = (int32_t)q;
+    cell[r] = c;
+    xy[r] = make_double4(cc.x, cc.y, dd.x, dd.y);
+    t[r] = row_t(seg, make_double2(cc.x, cc.y));
+  }
 };
 
 // query.py:357-455 locate_edge for segment q = (a, b): Cohen-Sutherland against the tree
@@ -399,18 +483,18 @@ struct HitRecord {
 // Segment-face queries on a tree with node boxes also skip subtrees by seg_meets_box.
 // `start` != 0: walk only the subtree of node `start`, which the caller reached by this same
 // walk's tests (a subquery, see ct_split.hip); the tree-bbox test was passed then.
-template <bool WRITE, int MAXV, int STACK, bool FACES>
-__device__ __forceinline__ int64_t edge_query(const TreeView& t, P2 a, P2 b, int64_t q, int64_t o,
-                                              int64_t* __restrict__ ii, int64_t* __restrict__ jj,
-                                              double* __restrict__ xy, LaneStack<STACK>& stack,
-                                              HitRecord rec = HitRecord{}, int start = 0) {
+// CAND (segment-face count pass only): a cell counts once Cohen-Sutherland accepts its bbox,
+// and the record keeps its leaf position k; the Cyrus-Beck clip of these candidates runs
+// afterwards, one candidate list per lane, without the walk's divergence around it.
+template <bool WRITE, int MAXV, int STACK, bool FACES, bool CAND = false, class OUT = FinalRows>
+__device__ __forceinline__ int64_t edge_query(const TreeView& t, P2 a, P2 b, int64_t q, int64_t o, const OUT& out,
+                                              LaneStack<STACK>& stack, HitRecord rec = HitRecord{}, int start = 0) {
   int64_t cnt = 0;
   P2 c0, d0;
   if (start == 0 && !cohen_sutherland(a, b, t.bbox, &c0, &d0)) return 0;
   const double vx = b.x - a.x, vy = b.y - a.y;
   // Non-finite segments keep the reference traversal unchanged (no pruning).
-  double mseg;
-  const double4* __restrict__ nbox = seg_prune_boxes<FACES>(t, a, b, vx, vy, &mseg);
+  const SegPrune pr = seg_prune<FACES>(t, a, b, vx, vy);
   // Same loop shape as locate_one: internal nodes in an inner loop, the leaf's cell tests
   // once every lane of the wavefront stands on a leaf or has finished.
   int sp = 0;
@@ -420,7 +504,7 @@ __device__ __forceinline__ int64_t edge_query(const TreeView& t, P2 a, P2 b, int
     bool exhausted = false;
     while (nd.child != -1) {
       bool left, right;
-      seg_children(nd, a, b, vx, vy, mseg, nbox, &left, &right);
+      seg_children(nd, a, b, vx, vy, pr, &left, &right);
       if (left && right) {
         stack.put(sp++, nd.child);
         node = nd.child + 1;
@@ -440,15 +524,23 @@ __device__ __forceinline__ int64_t edge_query(const TreeView& t, P2 a, P2 b, int
       const int c = t.bb_indices[k];
       P2 cc, dd;
       bool hit;
-      if constexpr (FACES) hit = edge_face_box<MAXV>(t, c, leaf_box(t, k, c), a, b, &cc, &dd);
-      else hit = edge_edge(t, c, a, b, &cc, &dd);
+      if constexpr (CAND) {
+        static_assert(FACES && !WRITE, "candidate lists are for segment-face count passes");
+        hit = cohen_sutherland(a, b, leaf_box(t, k, c), &cc, &dd);
+        if (hit && rec.cells != nullptr && cnt < rec.slots) rec.cells[cnt * rec.stride] = k;
+        cnt += hit ? 1 : 0;
+        continue;
+      } else if constexpr (FACES) {
+        hit = edge_face_box<MAXV>(t, c, leaf_box(t, k, c), a, b, &cc, &dd);
+      } else {
+        hit = edge_edge(t, c, a, b, &cc, &dd);
+      }
       if (hit) {
         if (WRITE) {
-          ii[o + cnt] = q;
-          jj[o + cnt] = c;
-          reinterpret_cast<double4*>(xy)[o + cnt] = make_double4(cc.x, cc.y, dd.x, dd.y);
+          out.put(o + cnt, q, c, cc, dd);
         } else if (rec.cells != nullptr && cnt < rec.slots) {
           rec.cells[cnt * rec.stride] = c;
+          if (cnt < rec.xy_slots) rec.xy[cnt * rec.stride] = make_double4(cc.x, cc.y, dd.x, dd.y);
         }
         ++cnt;
       }
@@ -461,32 +553,52 @@ __device__ __forceinline__ int64_t edge_query(const TreeView& t, P2 a, P2 b, int
 
 // geometry_utils.py:564-574 sort_intersections_by_edge on grouped rows, np.lexsort's order
 // (stable, NaN last).  One lane per row: the row's rank inside its segment's range = rows
-// with a smaller t plus earlier rows with an equal t; the row is written once, to its rank.
-// t = (c - a) . (b - a) with c the clipped start point, x-term + y-term as the reference's
-// np.sum over the last axis.
+// with a smaller t plus earlier rows with an equal t (row_t); the row is written once, to its
+// rank.
 __device__ __forceinline__ bool t_less(double x, double y) {
   return x < y || (!isnan(x) && isnan(y));
 }
 
-__device__ __forceinline__ void sort_row(const double* __restrict__ segs, const int64_t* __restrict__ offsets,
-                                         const int64_t* __restrict__ ii, const int64_t* __restrict__ jj_in,
-                                         const double* __restrict__ xy_in, int64_t r,
-                                         int64_t* __restrict__ jj_out, double* __restrict__ xy_out) {
-  const int64_t s = ii[r];
-  const int64_t lo = offsets[s], hi = offsets[s + 1];
-  const double4 s4 = reinterpret_cast<const double4*>(segs)[s];
-  const double abx = s4.z - s4.x, aby = s4.w - s4.y;
-  const double4 mine = reinterpret_cast<const double4*>(xy_in)[r];
-  const double t = (mine.x - s4.x) * abx + (mine.y - s4.y) * aby;
-  int64_t rank = lo;
-  for (int64_t k = lo; k < hi; ++k) {
-    if (k == r) continue;
-    const double2 c = reinterpret_cast<const double2*>(xy_in)[2 * k];
-    const double tk = (c.x - s4.x) * abx + (c.y - s4.y) * aby;
-    if (t_less(tk, t) || (k < r && !t_less(t, tk))) ++rank;
+// Rows are ranked a tile of kRankTile at a time by one 256-lane workgroup: the t of every
+// row within kRankHalo rows of the tile is computed once into LDS, and a row whose segment's
+// rows all lie in that window is ranked from LDS; a longer segment's row reads its
+// neighbours' start points from global memory.  Rows before r (equal t counts) and after r
+// (only smaller t counts) are two loops, so no lane tests k == r.
+constexpr int kRankTile = 1024;
+constexpr int kRankHalo = 256;
+constexpr int kRankWindow = kRankTile + 2 * kRankHalo;
+
+// Every lane of the workgroup calls this with the same r0 (it synchronises the workgroup);
+// tw holds kRankWindow doubles of LDS.
+__device__ __forceinline__ void sort_tile(const double* __restrict__ segs, const int64_t* __restrict__ offsets,
+                                          const int64_t* __restrict__ ii, const int64_t* __restrict__ jj_in,
+                                          const double* __restrict__ xy_in, int64_t n_rows, int64_t r0,
+                                          int64_t* __restrict__ jj_out, double* __restrict__ xy_out, double* tw) {
+  const double4* __restrict__ s4v = reinterpret_cast<const double4*>(segs);
+  const double2* __restrict__ xy2 = reinterpret_cast<const double2*>(xy_in);
+  const int64_t w0 = r0 > kRankHalo ? r0 - kRankHalo : 0;
+  const int64_t w1 = r0 + kRankTile + kRankHalo < n_rows ? r0 + kRankTile + kRankHalo : n_rows;
+  for (int64_t r = w0 + threadIdx.x; r < w1; r += blockDim.x) tw[r - w0] = row_t(s4v[ii[r]], xy2[2 * r]);
+  __syncthreads();
+  for (int k = threadIdx.x; k < kRankTile; k += blockDim.x) {
+    const int64_t r = r0 + k;
+    if (r >= n_rows) break;
+    const int64_t s = ii[r];
+    const int64_t lo = offsets[s], hi = offsets[s + 1];
+    const double t = tw[r - w0];
+    int64_t rank = lo;
+    if (lo >= w0 && hi <= w1) {
+      for (int64_t m = lo; m < r; ++m) rank += t_less(t, tw[m - w0]) ? 0 : 1;
+      for (int64_t m = r + 1; m < hi; ++m) rank += t_less(tw[m - w0], t) ? 1 : 0;
+    } else {
+      const double4 s4 = s4v[s];
+      for (int64_t m = lo; m < r; ++m) rank += t_less(t, row_t(s4, xy2[2 * m])) ? 0 : 1;
+      for (int64_t m = r + 1; m < hi; ++m) rank += t_less(row_t(s4, xy2[2 * m]), t) ? 1 : 0;
+    }
+    jj_out[rank] = jj_in[r];
+    reinterpret_cast<double4*>(xy_out)[rank] = reinterpret_cast<const double4*>(xy_in)[r];
   }
-  jj_out[rank] = jj_in[r];
-  reinterpret_cast<double4*>(xy_out)[rank] = mine;
+  __syncthreads();  // tw is refilled for the workgroup's next tile
 }
 
 }  // namespace

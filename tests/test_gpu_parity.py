@@ -720,14 +720,18 @@ def test_split_queries_are_exact(trees, max_parts, leaf_cells, slots):
         g.ORDER_MIN = 0  # the split path runs on ordered batches
         g.SPLIT_MIN_CELLS = 0 if split else 10**9
         g.SPLIT_MAX_PARTS = max_parts if split else 1
-        g.SPLIT_LEAF_CELLS = leaf_cells
+        g.SPLIT_LEAF_CELLS = g.EDGE_SPLIT_LEAF_CELLS = leaf_cells
         g.BOX_HIT_SLOTS = g.EDGE_HIT_SLOTS = max(slots, 1)
         g.BOX_HIT_BUDGET = g.EDGE_HIT_BUDGET = (slots * 10**6) if slots >= 4 else 0
+        # clipped-segment record: as many slots as cells, none, or fewer than the cells (the
+        # write pass then copies, clips recorded cells again, or walks again, per subquery)
+        g.EDGE_XY_SLOTS = {4: 4, 0: 0, 64: 16, 1024: 1024}[slots]
         try:
             return g.locate_boxes(boxes), g.intersect_edges(seg)
         finally:
-            for attr in ("ORDER_MIN", "SPLIT_MIN_CELLS", "SPLIT_MAX_PARTS", "SPLIT_LEAF_CELLS", "BOX_HIT_SLOTS",
-                         "EDGE_HIT_SLOTS", "BOX_HIT_BUDGET", "EDGE_HIT_BUDGET"):
+            for attr in ("ORDER_MIN", "SPLIT_MIN_CELLS", "SPLIT_MAX_PARTS", "SPLIT_LEAF_CELLS",
+                         "EDGE_SPLIT_LEAF_CELLS", "BOX_HIT_SLOTS",
+                         "EDGE_HIT_SLOTS", "BOX_HIT_BUDGET", "EDGE_HIT_BUDGET", "EDGE_XY_SLOTS"):
                 delattr(g, attr)
 
     (bi, bj), (ei, ej, exy) = run(True)
@@ -775,7 +779,7 @@ def test_split_edge_tree_is_exact():
     a = rng.uniform(-5, 105, size=(6000, 2))
     b = a + 60 * np.exp(rng.uniform(np.log(1e-3), 0, size=(6000, 1))) * rng.normal(size=(6000, 2))
     seg = np.stack([a, b], axis=1)
-    g.ORDER_MIN, g.SPLIT_MIN_CELLS, g.SPLIT_MAX_PARTS, g.SPLIT_LEAF_CELLS = 0, 0, 12, 2
+    g.ORDER_MIN, g.SPLIT_MIN_CELLS, g.SPLIT_MAX_PARTS, g.EDGE_SPLIT_LEAF_CELLS = 0, 0, 12, 2
     gi, gj, gxy = g.intersect_edges(seg)
     oi, oj, oxy = o.intersect_edges(seg)
     assert np.array_equal(gi, oi) and np.array_equal(gj, oj)
