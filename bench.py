@@ -205,13 +205,32 @@ def extras(args):
     pa = lambda: topo.pair_areas(sv_t, tv_t)  # noqa: E731
     pa()
     _, pa_ms = time_calls(pa, 20)
+    # the vertex backward on its own (k_pair_backward, the corner sums and the vertex gathers;
+    # HIP events on the launch stream), with the forward's outputs as saved by autograd
+    with torch.no_grad():
+        y_fb = rg(F, sv_t, tv_t)
+    tarea = topo.target_areas(tv_t)
+    vb = lambda: topo.vertex_backward(sv_t, tv_t, F, y_fb, G, tarea)  # noqa: E731
+    vb()
+    _, vb_ms = time_calls(vb, 20)
+    out["regrid_vertex_backward_16f_ms"] = vb_ms
     rc = counts["regrid"]
     if rc["kept"] == topo.n_pairs:
         out["regrid_roofline"] = fp64_roofline(
             "k_pair_areas (regrid forward clip)", rc["clip_flops_kept"], pa_ms,
             "tests/golden/flops_overlap.json regrid.clip_flops_kept: Sutherland-Hodgman + polygon_area ops "
             "of every kept pair (oracle-counted)")
-    del tree, rg, topo, W
+        # forward + backward: the forward's pair-area kernel plus the whole vertex backward;
+        # counted are the clip ops (forward) and, per pair, the 16-field dot product with its
+        # division and the overlap-area adjoint (backward); the backward's corner and vertex
+        # sums are not counted (a few adds per pair)
+        out["regrid_fwd_bwd_roofline"] = fp64_roofline(
+            "k_pair_areas + vertex backward (k_pair_backward, k_target_corners, k_source_corners, "
+            "k_vertex_gather)", rc["clip_flops_kept"] + rc["backward_pair_flops"] + rc["backward_ga_flops_16f"],
+            pa_ms + vb_ms,
+            "tests/golden/flops_overlap.json regrid: clip_flops_kept + backward_pair_flops (oracle "
+            "ref_pair_area_gradient) + backward_ga_flops_16f")
+    del tree, rg, topo, W, y_fb, tarea
     # compute_barycentric_weights with its adjoint: 1e6 points on the 20,000-triangle mesh
     # (points uniform over [-0.5, 10.5]^2; faces located once, then weights forward / backward).
     from numba_celltree_amd.barycentric import barycentric_weights

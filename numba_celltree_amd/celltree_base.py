@@ -422,9 +422,11 @@ class CellTree2dBase:
 
     def _split(self, kind: int, queries: torch.Tensor, order):
         """(subqueries (m, 2) int32, part offsets (n + 1), m) for kind 1 boxes, 2 segment-face,
-        3 segment-edge, or None when the batch is not split."""
+        3 segment-edge, or None when the batch is not split.  Batches too small to be ordered
+        (order None: queries in their own order) are split too: with few lanes, one long walk
+        would set the call's time."""
         n = int(queries.shape[0])
-        if order is None or self.depth > _native.CT_MAX_DEPTH or int(self.SPLIT_MAX_PARTS) <= 1:
+        if n == 0 or self.depth > _native.CT_MAX_DEPTH or int(self.SPLIT_MAX_PARTS) <= 1:
             return None
         st = D.stream()
         leaf = self.SPLIT_LEAF_CELLS if kind == 1 else self.EDGE_SPLIT_LEAF_CELLS

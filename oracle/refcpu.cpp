@@ -1132,6 +1132,109 @@ int n_threads() {
 // ===========================================================================
 // C ABI of the oracle (loaded by oracle/refcpu.py with ctypes)
 // ===========================================================================
+// ------------------------------------------------------------ overlap-area adjoint
+// No reference counterpart (the reference computes no derivatives): the derivative of
+// polygon_polygon_clip_area (sutherland_hodgman.py:84-148) of two convex counter-clockwise
+// polygons a (subject) and b (clipper) with respect to their vertices, restated from the
+// mathematics for the roofline and the checks of the GPU regrid backward.  The overlap P is
+// bounded by the pieces of the input edges that lie in the other polygon; moving the ends of
+// edge v0 -> v1 (d = v1 - v0) moves the point at parameter u by (1 - u) and u times the
+// displacement, and the area changes by the outward normal velocity integrated over the piece
+// [alpha, beta] of the edge inside the other polygon:
+//   dA/dv0 = m ((beta - alpha) - (beta^2 - alpha^2) / 2),   dA/dv1 = m (beta^2 - alpha^2) / 2,
+// with m = (d.y, -d.x).  A piece is an edge clipped by the other polygon's half-planes
+// (inside: u x (p - r) > 0, the clip's test): subject edges by open half-planes, clipper edges
+// by closed ones, where a clipper edge on a subject edge line counts when both run the same
+// way (the clip keeps such a boundary piece as the clipper's).  Bounds are fractions
+// (numerator / positive denominator) compared by cross-multiplication, each end divided once.
+// Operations are counted as in the forward (flops<>), the cross-multiplications included.
+struct PieceRange {
+  double n0 = 0.0, d0 = 1.0, n1 = 1.0, d1 = 1.0;
+  bool empty = false;
+};
+
+template <bool COUNT>
+void clip_piece(Point v0, double dx, double dy, Point r, Point s, bool closed, PieceRange& e) {
+  const double ux = s.x - r.x, uy = s.y - r.y;
+  const double c0 = ux * (v0.y - r.y) - uy * (v0.x - r.x);
+  const double c1 = ux * dy - uy * dx;
+  flops<COUNT>(2 + 5 + 3);
+  if (c1 == 0.0) {
+    bool in = c0 > 0.0;
+    if (closed && c0 == 0.0) {
+      in = (ux * dx + uy * dy) > 0.0;
+      flops<COUNT>(3);
+    }
+    e.empty = e.empty || !in;
+  } else if (c1 > 0.0) {  // t >= -c0 / c1
+    const double num = -c0;
+    flops<COUNT>(2);
+    if (num * e.d0 > e.n0 * c1) {
+      e.n0 = num;
+      e.d0 = c1;
+    }
+  } else {  // t <= c0 / -c1
+    const double den = -c1;
+    flops<COUNT>(2);
+    if (c0 * e.d1 < e.n1 * den) {
+      e.n1 = c0;
+      e.d1 = den;
+    }
+  }
+}
+
+// The two end contributions of edge v0 -> v1's piece inside polygon c (lc vertices), scaled
+// by gA; false when the piece is empty.  clipper_edges: v0 -> v1 is an edge of the clipper
+// and c the subject (closed half-planes).
+template <bool COUNT>
+bool piece_gradient(Point v0, Point v1, const Point* c, int lc, bool clipper_edges, double gA, Vector* g0,
+                    Vector* g1) {
+  const double dx = v1.x - v0.x, dy = v1.y - v0.y;
+  flops<COUNT>(2);
+  PieceRange e;
+  for (int j = 0; j < lc; ++j) {
+    // subject edges meet the clipper's edges c[j - 1] -> c[j]; clipper edges the subject's
+    // c[j] -> c[j + 1] (the GPU kernel's enumeration, so that equal bounds resolve alike)
+    const Point r = clipper_edges ? c[j] : c[(j + lc - 1) % lc];
+    const Point s = clipper_edges ? c[(j + 1) % lc] : c[j];
+    clip_piece<COUNT>(v0, dx, dy, r, s, clipper_edges, e);
+  }
+  const double alpha = e.n0 / e.d0, beta = e.n1 / e.d1;
+  flops<COUNT>(2);
+  if (e.empty || !(beta > alpha)) return false;
+  const double w1 = 0.5 * (beta * beta - alpha * alpha);
+  const double w0 = (beta - alpha) - w1;
+  *g0 = {gA * w0 * dy, -gA * w0 * dx};
+  *g1 = {gA * w1 * dy, -gA * w1 * dx};
+  flops<COUNT>(4 + 2 + 4 + 4);
+  return true;
+}
+
+// Pair gradient: ga[i] = gA dA/da[i], gb[j] = gA dA/db[j] (la, lb vertices), summed edge by
+// edge (4 adds per contributing edge).
+template <bool COUNT>
+void pair_area_gradient(const Point* a, int la, const Point* b, int lb, double gA, Vector* ga, Vector* gb) {
+  for (int i = 0; i < la; ++i) ga[i] = {0.0, 0.0};
+  for (int j = 0; j < lb; ++j) gb[j] = {0.0, 0.0};
+  Vector g0, g1;
+  for (int i = 0; i < la; ++i) {
+    const int i1 = (i + 1) % la;
+    if (piece_gradient<COUNT>(a[i], a[i1], b, lb, false, gA, &g0, &g1)) {
+      ga[i] = {ga[i].x + g0.x, ga[i].y + g0.y};
+      ga[i1] = {ga[i1].x + g1.x, ga[i1].y + g1.y};
+      flops<COUNT>(4);
+    }
+  }
+  for (int j = 0; j < lb; ++j) {
+    const int j0 = (j + lb - 1) % lb;
+    if (piece_gradient<COUNT>(b[j0], b[j], a, la, true, gA, &g0, &g1)) {
+      gb[j0] = {gb[j0].x + g0.x, gb[j0].y + g0.y};
+      gb[j] = {gb[j].x + g1.x, gb[j].y + g1.y};
+      flops<COUNT>(4);
+    }
+  }
+}
+
 extern "C" {
 
 struct ref_tree_t {
@@ -1339,6 +1442,35 @@ void ref_face_pair_flops(const double* va, const double* vb, const int64_t* fa, 
     t_flops = 0;
     area[i] = ok ? polygon_polygon_clip_area<true>(a, la, b, lb) : 0.0;
     clip[i] = t_flops;
+  }
+}
+
+// Per pair i (subject a = faces_a[ia[i]], clipper b = faces_b[ib[i]], both convex and
+// counter-clockwise): the overlap-area adjoint scaled by gA[i] (see pair_area_gradient) in
+// ga[i * na_max + slot] / gb[i * nb_max + slot] (x, y per vertex slot; unused slots 0), and
+// its FP64 operation count in ops[i] (0 where gA[i] == 0: no gradient is formed).
+void ref_pair_area_gradient(const double* va, const double* vb, const int64_t* fa, int na_max,
+                            const int64_t* fb, int nb_max, const int64_t* ia, const int64_t* ib, int64_t n,
+                            const double* gA, double* ga, double* gb, int64_t* ops) {
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int64_t i = 0; i < n; ++i) {
+    Point a[MAX_N_VERTEX], b[MAX_N_VERTEX];
+    Vector da[MAX_N_VERTEX], db[MAX_N_VERTEX];
+    const int la = copy_vertices(va, fa + ia[i] * na_max, na_max, a);
+    const int lb = copy_vertices(vb, fb + ib[i] * nb_max, nb_max, b);
+    for (int k = 0; k < na_max; ++k) da[k] = {0.0, 0.0};
+    for (int k = 0; k < nb_max; ++k) db[k] = {0.0, 0.0};
+    t_flops = 0;
+    if (gA[i] != 0.0) pair_area_gradient<true>(a, la, b, lb, gA[i], da, db);
+    ops[i] = t_flops;
+    for (int k = 0; k < na_max; ++k) {
+      ga[2 * (i * na_max + k)] = da[k].x;
+      ga[2 * (i * na_max + k) + 1] = da[k].y;
+    }
+    for (int k = 0; k < nb_max; ++k) {
+      gb[2 * (i * nb_max + k)] = db[k].x;
+      gb[2 * (i * nb_max + k) + 1] = db[k].y;
+    }
   }
 }
 

@@ -153,6 +153,7 @@ def lib():
             "ref_polygons_intersect": (None, [P, P, P, i32, P, i32, P, P, i64, P]),
             "ref_area_of_intersection": (None, [P, P, P, i32, P, i32, P, P, i64, P]),
             "ref_face_pair_flops": (None, [P, P, P, i32, P, i32, P, P, i64, P, P, P, P]),
+            "ref_pair_area_gradient": (None, [P, P, P, i32, P, i32, P, P, i64, P, P, P, P]),
             "ref_box_area_of_intersection": (None, [P, P, P, i32, P, P, i64, P]),
             "ref_barycentric_triangle_weights": (None, [P, P, i64, P, i32, P, f64, P]),
             "ref_barycentric_wachspress_weights": (None, [P, P, i64, P, i32, P, f64, P]),
@@ -333,6 +334,25 @@ def face_pair_flops(vertices_a, vertices_b, faces_a, faces_b, indices_a, indices
         _p(ia), _p(ib), n, _p(sat), _p(passed), _p(clip), _p(area),
     )
     return sat, passed.astype(bool), clip, area
+
+
+def pair_area_gradient(vertices_a, vertices_b, faces_a, faces_b, indices_a, indices_b, g_area):
+    """Per pair (subject faces_a[indices_a], clipper faces_b[indices_b], convex, CCW): g_area
+    times the derivative of area_of_intersection with respect to the subject's and the
+    clipper's vertices, (ga (n, na_max, 2), gb (n, nb_max, 2)) by vertex slot, and the FP64
+    operations of each (ref_pair_area_gradient; no reference counterpart)."""
+    fa, fb = _i64(faces_a), _i64(faces_b)
+    ia, ib = _i64(indices_a), _i64(indices_b)
+    n = len(ia)
+    g = np.ascontiguousarray(np.broadcast_to(np.asarray(g_area, dtype=np.float64), (n,)))
+    ga = np.zeros((n, fa.shape[1], 2))
+    gb = np.zeros((n, fb.shape[1], 2))
+    ops = np.empty(n, dtype=np.int64)
+    lib().ref_pair_area_gradient(
+        _p(_f64(vertices_a)), _p(_f64(vertices_b)), _p(fa), fa.shape[1], _p(fb), fb.shape[1],
+        _p(ia), _p(ib), n, _p(g), _p(ga), _p(gb), _p(ops),
+    )
+    return ga, gb, ops
 
 
 def box_area_of_intersection(bbox_coords, vertices, faces, indices_bbox, indices_face):

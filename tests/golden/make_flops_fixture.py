@@ -8,7 +8,11 @@ counter against an independent Python count):
   (targets), the pair stage of celltree.py:248-269: SAT on every bbox candidate pair, then
   the Sutherland-Hodgman clip and its polygon area on every pair SAT passes;
 * regrid: the 999,941-cell mixed mesh -> the same 250,000 targets; the forward recomputes
-  the clip area of every kept pair (area > 0), clip_flops_kept.
+  the clip area of every kept pair (area > 0), clip_flops_kept; the backward's pair kernel
+  forms, per kept pair, the field dot product of 16 fields and its division by the target
+  area (2 * 16 + 1 operations, backward_ga_flops_16f) and the overlap-area adjoint
+  (backward_pair_flops, counted by the oracle's ref_pair_area_gradient, which has no
+  reference counterpart: tests/test_oracle_adjoint.py pins it).
 
 Run:  python tests/golden/make_flops_fixture.py   (CPU oracle, about a minute on 8 cores)
 """
@@ -28,7 +32,7 @@ if ROOT not in sys.path:
 OUT = os.path.join(HERE, "flops_overlap.json")
 
 
-def _counts(sv, sf, tv, tf):
+def _counts(sv, sf, tv, tf, backward_fields=0):
     from oracle import refcpu as R
 
     tf = tf.astype(np.int64).copy()
@@ -37,7 +41,7 @@ def _counts(sv, sf, tv, tf):
     ii, jj = tree.locate_boxes(R.build_face_bboxes(tf, tv))
     sat, passed, clip, area = R.face_pair_flops(tv, sv, tf, tree.faces, ii, jj)
     kept = area > 0
-    return {
+    out = {
         "candidates": int(len(ii)),
         "sat_passed": int(passed.sum()),
         "kept": int(kept.sum()),
@@ -45,6 +49,12 @@ def _counts(sv, sf, tv, tf):
         "clip_flops": int(clip.sum()),
         "clip_flops_kept": int(clip[kept].sum()),
     }
+    if backward_fields:
+        # every kept pair has a nonzero scale (random fields and output gradients)
+        _, _, ops = R.pair_area_gradient(tv, sv, tf, tree.faces, ii[kept], jj[kept], 1.0)
+        out["backward_pair_flops"] = int(ops.sum())
+        out[f"backward_ga_flops_{backward_fields}f"] = int(kept.sum()) * (2 * backward_fields + 1)
+    return out
 
 
 def overlap_counts():
@@ -59,13 +69,14 @@ def regrid_counts():
 
     sv, sf = S.config2_mesh()
     tv, tf = S.hex_voronoi(500, 500, 1000.0, 0.1, 7)
-    return _counts(sv, sf, tv, tf)
+    return _counts(sv, sf, tv, tf, backward_fields=16)
 
 
 def main():
     t0 = time.perf_counter()
     doc = {
-        "what": "FP64 add/sub/mul/div counts of the executed reference path (oracle ref_face_pair_flops)",
+        "what": "FP64 add/sub/mul/div counts of the executed reference path (oracle ref_face_pair_flops) "
+                "and of the overlap-area adjoint (oracle ref_pair_area_gradient)",
         "intersect_faces": {"workload": "999,698 jittered triangles x 250,000 hex-Voronoi cells", **overlap_counts()},
         "regrid": {"workload": "999,941-cell mixed mesh -> 250,000 hex-Voronoi cells", **regrid_counts()},
     }

@@ -287,3 +287,6 @@ def test_fixture_counts_are_reproducible():
     got = mk.overlap_counts()
     for key in ("candidates", "sat_passed", "kept", "sat_flops", "clip_flops", "clip_flops_kept"):
         assert got[key] == want["intersect_faces"][key], key
+    got = mk.regrid_counts()
+    for key in ("kept", "clip_flops_kept", "backward_pair_flops", "backward_ga_flops_16f"):
+        assert got[key] == want["regrid"][key], key
