@@ -131,8 +131,8 @@ def extras(args):
 
     out = {}
     counts = flop_counts()
-    v, f = S.config2_mesh()
-    _, bx, sg = S.config2_queries(10, args.boxes, args.segments)
+    v, f = S.mixed_mesh_1m()
+    _, bx, sg = S.mixed_mesh_queries(10, args.boxes, args.segments)
     tree = M.CellTree2d(torch.from_numpy(v).cuda(), torch.from_numpy(f).cuda(), -1)
     d_bx = torch.from_numpy(bx).cuda()
     d_sg = torch.from_numpy(sg).cuda()
@@ -235,7 +235,7 @@ def extras(args):
     # (points uniform over [-0.5, 10.5]^2; faces located once, then weights forward / backward).
     from numba_celltree_amd.barycentric import barycentric_weights
 
-    v1, f1, _ = S.config1()
+    v1, f1, _ = S.triangle_grid_20k()
     tri = M.CellTree2d(torch.from_numpy(v1).cuda(), torch.from_numpy(f1).cuda(), -1)
     bp = torch.from_numpy(np.random.default_rng(4).uniform(-0.5, 10.5, size=(1_000_000, 2))).cuda()
     Gb = torch.randn(1_000_000, 3, dtype=torch.float64, device="cuda")
@@ -257,7 +257,7 @@ def extras(args):
         wall, _ = time_calls(fn, 10)
         out[name] = 1e3 * wall / 10
     del tri, bp, Gb
-    sv, sf, tv, tf, _ = S.config3()
+    sv, sf, tv, tf, _ = S.overlap_meshes()
     for label in ("build_999698_tris_cold_ms", "build_999698_tris_ms"):
         t0 = time.perf_counter()
         src = M.CellTree2d(torch.from_numpy(sv).cuda(), torch.from_numpy(sf).cuda(), -1)
@@ -317,7 +317,7 @@ def batch_extras(args):
 
     out = {}
     t0 = time.perf_counter()
-    sets = S.config5(n_meshes=args.batch_meshes)
+    sets = S.batch_meshes(n_meshes=args.batch_meshes)
     log(f"batch workload generated in {time.perf_counter() - t0:.1f} s")
     dev = torch.device("cuda")
     t0 = time.perf_counter()
@@ -450,8 +450,8 @@ def main() -> int:
         print(json.dumps({"batch": batch_extras(args)}), flush=True)
         return 0
     log("generating workload")
-    v, f = S.config2_mesh()
-    pts, _, _ = S.config2_queries(args.points, 10, 10)
+    v, f = S.mixed_mesh_1m()
+    pts, _, _ = S.mixed_mesh_queries(args.points, 10, 10)
     d_v = torch.from_numpy(v).cuda()
     d_f = torch.from_numpy(f).cuda()
     d_pts = torch.from_numpy(pts).cuda()
@@ -508,7 +508,7 @@ def main() -> int:
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 5),
-        "traffic": read_traffic("locate_points_cfg2_traffic.txt"),
+        "traffic": read_traffic("locate_points_1e7_traffic.txt"),
         "kernel": "k_locate_points (traversal)",
         "kernel_ms": round(kernel_ms, 4),
         "order_ms": round(order_ms, 4),

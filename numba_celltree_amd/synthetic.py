@@ -165,21 +165,24 @@ def sinusoid_fields(xy: np.ndarray, n_fields: int, seed: int) -> np.ndarray:
     return out
 
 
-# ------------------------------------------------------------------- named configs
-def config1(seed: int = 0):
+# ------------------------------------------------------------------- named workloads
+def triangle_grid_20k(seed: int = 0):
     """20,000 triangles on [0,10]^2 (101x101 vertices, jitter 0.3h); 1e5 points over [-0.5,10.5]^2."""
     v, f = jittered_grid(101, 10.0, 0.3, 1.0, seed)
     p = _rng(seed, 50).uniform(-0.5, 10.5, size=(100_000, 2))
     return v, f, p
 
 
-def config2_mesh(nv: int = 867, seed: int = 0):
+def mixed_mesh_1m(nv: int = 867, seed: int = 0):
     """Mixed mesh: nv x nv vertices on [0,1000]^2, jitter 0.25h, a third of the quads split
     (867 -> 999,941 cells: 499,970 triangles + 499,971 quads), faces int32 width 4."""
     return jittered_grid(nv, 1000.0, 0.25, 1.0 / 3.0, seed)
 
 
-def config2_queries(n_points: int, n_boxes: int, n_segments: int, nv: int = 867, seed: int = 0):
+def mixed_mesh_queries(n_points: int, n_boxes: int, n_segments: int, nv: int = 867, seed: int = 0):
+    """Queries on the mixed mesh's domain: points (95% inside, the rest in a 50-wide frame),
+    boxes (centres uniform, widths log-uniform in [0.1h, 20h]) and segments (uniform
+    start, length log-uniform in [0.1h, 50h], uniform angle)."""
     h = 1000.0 / (nv - 1)
     return (
         points(n_points, 1000.0, 0.95, 50.0, seed),
@@ -188,7 +191,7 @@ def config2_queries(n_points: int, n_boxes: int, n_segments: int, nv: int = 867,
     )
 
 
-def config3(nv: int = 708, n_target_side: int = 500, seed: int = 0):
+def overlap_meshes(nv: int = 708, n_target_side: int = 500, seed: int = 0):
     """Source: nv x nv vertices on [0,1000]^2, jitter 0.25h, all quads split (708 -> 999,698
     triangles).  Target: n_target_side^2 hex-lattice Voronoi cells (500 -> 250,000, 3-6
     vertices after clipping).  Field: one sum of 8 sinusoids at source centroids (seed 1)."""
@@ -199,7 +202,7 @@ def config3(nv: int = 708, n_target_side: int = 500, seed: int = 0):
     return sv, sf, tv, tf, field
 
 
-def config5(n_meshes: int = 512, seed: int = 0, cells=(1e3, 2e5), n_points=(1e3, 1e6),
+def batch_meshes(n_meshes: int = 512, seed: int = 0, cells=(1e3, 2e5), n_points=(1e3, 1e6),
             n_segments=(1e2, 1e5)):
     """Batch of independent meshes.  Mesh m: cell count log-uniform in `cells` (jittered grid,
     jitter 0.25h, a third of the quads split: half triangles, half quads, faces width 4) on its

@@ -60,14 +60,14 @@ def _counts(sv, sf, tv, tf, backward_fields=0):
 def overlap_counts():
     from numba_celltree_amd import synthetic as S
 
-    sv, sf, tv, tf, _ = S.config3()
+    sv, sf, tv, tf, _ = S.overlap_meshes()
     return _counts(sv, sf, tv, tf)
 
 
 def regrid_counts():
     from numba_celltree_amd import synthetic as S
 
-    sv, sf = S.config2_mesh()
+    sv, sf = S.mixed_mesh_1m()
     tv, tf = S.hex_voronoi(500, 500, 1000.0, 0.1, 7)
     return _counts(sv, sf, tv, tf, backward_fields=16)
 

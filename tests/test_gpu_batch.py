@@ -24,9 +24,9 @@ def log(msg):
 def meshes():
     import numba_celltree_amd as M
 
-    sets = S.config5(n_meshes=14, seed=3, cells=(200, 6000), n_points=(50, 20_000), n_segments=(10, 3000))
+    sets = S.batch_meshes(n_meshes=14, seed=3, cells=(200, 6000), n_points=(50, 20_000), n_segments=(10, 3000))
     # a triangle-only mesh (3 columns) among the mixed ones, an empty and a one-query set
-    v, f, p = S.config1(seed=2)
+    v, f, p = S.triangle_grid_20k(seed=2)
     seg = S.segments(500, 10.0, 0.1, 0.1, 50.0, 2)
     sets.append((v, f, p[:7000], seg))
     sets[2] = (sets[2][0], sets[2][1], sets[2][2][:0], sets[2][3][:1])

@@ -30,8 +30,8 @@ def full():
     import numba_celltree_amd as M
 
     log("generating the mesh and queries")
-    v, f = S.config2_mesh()
-    pts, bx, sg = S.config2_queries(10_000_000, 1_000_000, 1_000_000)
+    v, f = S.mixed_mesh_1m()
+    pts, bx, sg = S.mixed_mesh_queries(10_000_000, 1_000_000, 1_000_000)
     log("building the trees")
     g = M.CellTree2d(torch.from_numpy(v).cuda(), torch.from_numpy(f).cuda(), -1)
     o = R.OracleCellTree2d(v, f, -1)
@@ -93,7 +93,7 @@ def test_full_intersect_faces():
     import numba_celltree_amd as M
 
     log("intersect_faces 999,698 x 250,000")
-    sv, sf, tv, tf, _ = S.config3()
+    sv, sf, tv, tf, _ = S.overlap_meshes()
     g = M.CellTree2d(torch.from_numpy(sv).cuda(), torch.from_numpy(sf).cuda(), -1)
     o = R.OracleCellTree2d(sv, sf, -1)
     assert np.array_equal(g.bb_indices, o.bb_indices)
@@ -137,7 +137,7 @@ def test_full_barycentric_weights():
     import numba_celltree_amd as M
 
     log("barycentric 1e6 points")
-    v, f, _ = S.config1()
+    v, f, _ = S.triangle_grid_20k()
     pts = np.random.default_rng(4).uniform(-0.5, 10.5, size=(1_000_000, 2))
     gf, gw = M.CellTree2d(v, f, -1).compute_barycentric_weights(pts)
     of, ow = R.OracleCellTree2d(v, f, -1).compute_barycentric_weights(pts)
@@ -159,7 +159,7 @@ def test_full_barycentric_adjoint():
     from numba_celltree_amd.barycentric import BarycentricFunction, barycentric_weights
 
     log("barycentric adjoint 1e6 points")
-    v, f, _ = S.config1()
+    v, f, _ = S.triangle_grid_20k()
     tree = M.CellTree2d(v, f, -1)
     rng = np.random.default_rng(4)
     pts_h = rng.uniform(-0.5, 10.5, size=(1_000_000, 2))

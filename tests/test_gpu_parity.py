@@ -2,7 +2,7 @@
 
 Bar: integer and index results bit-exact, in the reference's order (the oracle
 reproduces the reference's chunk-concatenated order, so ordered arrays are compared, which
-implies the north star's sorted-set equality); FP64 results (areas, clipped points,
+implies equality of the sorted (query, cell) sets); FP64 results (areas, clipped points,
 weights) within 1e-12 relative, measured against max(|value|, scale).
 """
 
@@ -47,8 +47,8 @@ def same_tree(gpu, ora):
 
 
 MESHES = {
-    "cfg1_tri20k": lambda: S.config1()[:2],
-    "mixed_45k": lambda: S.config2_mesh(nv=151, seed=3),
+    "tri20k": lambda: S.triangle_grid_20k()[:2],
+    "mixed_45k": lambda: S.mixed_mesh_1m(nv=151, seed=3),
     "tri_voronoi": lambda: S.hex_voronoi(60, 50, 100.0, 0.1, 5),
 }
 
@@ -195,7 +195,7 @@ def test_start_grid_nodes_are_forced():
     for sampled points of each cell, the oracle-side forced descent passes through it."""
     from numba_celltree_amd import CellTree2d
 
-    v, f = S.config1()[:2]
+    v, f = S.triangle_grid_20k()[:2]
     g = CellTree2d(v, f, -1)
     n = 16
     grid = g.d_start_grid

@@ -275,7 +275,7 @@ def test_barycentric_adjoint():
     import numba_celltree_amd as M
     from numba_celltree_amd.barycentric import barycentric_weights
 
-    v, f, _ = S.config1()
+    v, f, _ = S.triangle_grid_20k()
     tree = M.CellTree2d(v, f, -1)
     rng = np.random.default_rng(4)
     pts = torch.from_numpy(rng.uniform(0.2, 9.8, size=(20_000, 2))).cuda()
@@ -353,7 +353,7 @@ def test_inputs_are_checked_or_converted():
     with pytest.raises(ValueError):
         rg(F64, tgt_vertices=topo.tgt_vertices[:, :1])
 
-    v, f, _ = S.config1()
+    v, f, _ = S.triangle_grid_20k()
     tri = M.CellTree2d(v, f, -1)
     pts = torch.from_numpy(np.random.default_rng(2).uniform(0.5, 9.5, (1000, 2))).cuda()
     face, w = barycentric_weights(tri, pts)

@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _meshes():
-    v, f, _ = S.config1()
+    v, f, _ = S.triangle_grid_20k()
     yield "triangles", v, f
     v, f = S.jittered_grid(151, 1000.0, 0.25, 1.0 / 3.0, 5)
     yield "mixed", v, f
@@ -113,7 +113,7 @@ def test_edge_tree_round_trip():
 def test_import_rejects_inconsistent_boxes():
     import numba_celltree_amd as M
 
-    v, f, _ = S.config1()
+    v, f, _ = S.triangle_grid_20k()
     a = M.CellTree2d(v, f, -1).to_arrays()
     a["bb_coords"] = a["bb_coords"].copy()
     a["bb_coords"][17, 1] += 1.0

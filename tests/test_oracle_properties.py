@@ -26,7 +26,7 @@ from oracle import refcpu as R
 
 @pytest.fixture(scope="module")
 def small_mixed():
-    v, f = S.config2_mesh(nv=41, seed=11)
+    v, f = S.mixed_mesh_1m(nv=41, seed=11)
     return v, f, R.OracleCellTree2d(v, f, -1)
 
 
@@ -37,7 +37,7 @@ def polys_of(v, f):
 # ------------------------------------------------------------------ tree build
 @pytest.mark.parametrize("nb,cpl", [(4, 2), (2, 1), (8, 3)])
 def test_tree_invariants(nb, cpl):
-    v, f = S.config2_mesh(nv=61, seed=2)
+    v, f = S.mixed_mesh_1m(nv=61, seed=2)
     t = R.OracleCellTree2d(v, f, -1, n_buckets=nb, cells_per_leaf=cpl)
     nodes, bbi, bb = t.nodes, t.bb_indices, t.bb_coords
     assert np.array_equal(np.sort(bbi), np.arange(len(f)))

@@ -30,7 +30,7 @@ def _arrays(tree, kind="faces"):
 
 @pytest.fixture(scope="module")
 def oracle_tree():
-    v, f, _ = S.config1()
+    v, f, _ = S.triangle_grid_20k()
     return R.OracleCellTree2d(v, f, -1)
 
 

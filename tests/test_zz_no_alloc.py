@@ -73,7 +73,7 @@ def test_noalloc_rejects_oversized_input():
     from numba_celltree_amd import _device as D
     from numba_celltree_amd import _native
 
-    v, f, _ = S.config1()
+    v, f, _ = S.triangle_grid_20k()
     tree = M.CellTree2d(v, f, -1)
     n = _native.CT_NOALLOC_MAX_POINTS + 1
     pts = torch.zeros((n, 2), dtype=torch.float64, device="cuda")
