@@ -692,6 +692,38 @@ def test_locate_boxes_hit_record_is_exact(trees):
         assert np.array_equal(c, c0) and np.array_equal(i, i0) and np.array_equal(j, j0), slots
 
 
+def test_small_and_degenerate_box_and_segment_batches(trees):
+    """Box and segment batches at the edges of the split path, against the oracle: no
+    queries, one query (a segment across the mesh and a box over most of it, both split),
+    queries that all miss the tree, and non-finite queries among others."""
+    name, v, f, g, o = trees
+    L = _extent(v)
+    lo = v.min(0)
+    log(f"small / degenerate batches {name}")
+    mid = lo + 0.5 * L
+    cases = {
+        "empty": (np.zeros((0, 4)), np.zeros((0, 2, 2))),
+        "one": (np.array([[lo[0] + 0.05 * L, lo[0] + 0.9 * L, lo[1] + 0.1 * L, lo[1] + 0.8 * L]]),
+                np.array([[[lo[0] + 0.01 * L, lo[1] + 0.02 * L], [lo[0] + 0.97 * L, lo[1] + 0.93 * L]]])),
+        "outside": (np.array([[lo[0] - 3 * L, lo[0] - 2 * L, lo[1], lo[1] + L]] * 40),
+                    np.array([[[lo[0] - 3 * L, lo[1]], [lo[0] - 2 * L, lo[1] + L]]] * 40)),
+        "non-finite": (np.array([[np.nan, mid[0], lo[1], mid[1]], [lo[0], np.inf, lo[1], mid[1]],
+                                 [lo[0], mid[0], lo[1], mid[1]]] * 10),
+                       np.array([[[np.nan, mid[1]], [mid[0], mid[1]]], [[lo[0], lo[1]], [np.inf, mid[1]]],
+                                 [[lo[0], lo[1]], [mid[0], mid[1]]]] * 10)),
+    }
+    for label, (boxes, segs) in cases.items():
+        gi, gj = g.locate_boxes(boxes)
+        oi, oj = o.locate_boxes(boxes)
+        assert np.array_equal(gi, oi) and np.array_equal(gj, oj), label
+        gi, gj, gxy = g.intersect_edges(segs)
+        oi, oj, oxy = o.intersect_edges(segs)
+        assert np.array_equal(gi, oi) and np.array_equal(gj, oj), label
+        assert_close(gxy.reshape(-1, 4), oxy.reshape(-1, 4), L)
+        if label == "one":
+            assert len(gi) > 10
+
+
 @pytest.mark.parametrize("max_parts, leaf_cells, slots", [(2, 2, 4), (7, 4, 0), (16, 1, 64), (16, 16, 1024)])
 def test_split_queries_are_exact(trees, max_parts, leaf_cells, slots):
     """Box and segment queries cut into subqueries by subtree (ct_split_queries) give the
