@@ -518,7 +518,7 @@ __global__ void __launch_bounds__(kBlock) k_edges_cand(TreeView t, const double*
 // its owner's hits so far plus the hits among the owner's earlier candidates of the same
 // batch (a ballot).  The lists are compacted in place: a hit's slot is at most its own
 // candidate's slot, and every candidate of a batch is read before any hit of it is written.
-template <int MAXV>
+template <int MAXV, bool ROWS>
 __global__ void __launch_bounds__(kBlock) k_edges_clip(TreeView t, const double* __restrict__ segs,
                                                        const ct_subquery* __restrict__ subs, int64_t n,
                                                        int64_t* __restrict__ counts, int32_t* __restrict__ hits,
@@ -563,7 +563,8 @@ __global__ void __launch_bounds__(kBlock) k_edges_clip(TreeView t, const double*
     if (g < total) {
       const int k = hits[(int64_t)(g - oexcl) * n + oi];
       c = t.bb_indices[k];
-      hit = edge_face_clip_leaf<MAXV>(t, k, c, leaf_box(t, k, c), a, b, &cc, &dd);
+      if constexpr (ROWS) hit = edge_face_clip_row<MAXV>(t, k, c, a, b, &cc, &dd);
+      else hit = edge_face_clip<MAXV>(t, c, leaf_box(t, k, c), a, b, &cc, &dd);
     }
     const uint64_t mask = __ballot(hit);
     const int first = oexcl > base ? oexcl - base : 0;  // batch lane of the owner's first candidate
@@ -655,7 +656,10 @@ struct EdgesSubLaunch {
       if (FACES && hits != nullptr) {
         k_edges_cand<MAXV, S><<<g, kBlock, 0, st>>>(t, segs, subs, n, counts, hits, slots);
         CT_LAUNCH_CHECK();
-        k_edges_clip<MAXV><<<g, kBlock, 0, st>>>(t, segs, subs, n, counts, hits, slots, hit_xy, xy_slots);
+        if (t.leaf != nullptr)
+          k_edges_clip<MAXV, true><<<g, kBlock, 0, st>>>(t, segs, subs, n, counts, hits, slots, hit_xy, xy_slots);
+        else
+          k_edges_clip<MAXV, false><<<g, kBlock, 0, st>>>(t, segs, subs, n, counts, hits, slots, hit_xy, xy_slots);
         CT_LAUNCH_CHECK();
         k_edges_overflow<MAXV, S><<<g, kBlock, 0, st>>>(t, segs, subs, n, counts, hits, slots, hit_xy, xy_slots);
         CT_LAUNCH_CHECK();

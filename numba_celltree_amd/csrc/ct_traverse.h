@@ -279,13 +279,13 @@ __device__ __forceinline__ bool edge_face_clip(const TreeView& t, int c, const B
   return cyrus_beck<MAXV>(a, b, poly, len, tol, cc, dd);
 }
 
-// edge_face_clip for the cell at leaf position k: its vertices from the packed leaf rows (a
-// read that does not wait for c; slots past the polygon's length are never read), only the
-// length from its face row.
+// edge_face_clip for the cell at leaf position k of a tree with packed leaf rows: its vertices
+// from the row (a read that does not wait for c; slots past the polygon's length repeat its
+// last vertex and are never read by the clip), its length from its face row, and the cell box
+// behind the tolerance as the min / max of the row's vertices -- the values the face bboxes
+// hold, so the same tolerance bit for bit, without loading the box.
 template <int MAXV>
-__device__ __forceinline__ bool edge_face_clip_leaf(const TreeView& t, int k, int c, const Box& box, P2 a, P2 b,
-                                                    P2* cc, P2* dd) {
-  if (t.leaf == nullptr) return edge_face_clip<MAXV>(t, c, box, a, b, cc, dd);
+__device__ __forceinline__ bool edge_face_clip_row(const TreeView& t, int k, int c, P2 a, P2 b, P2* cc, P2* dd) {
   P2 poly[MAXV];
   load_leaf_poly<MAXV, false>(t.leaf, t.nmax, k, poly);
   const int32_t* row = t.elements + (int64_t)c * t.nmax;
@@ -293,7 +293,15 @@ __device__ __forceinline__ bool edge_face_clip_leaf(const TreeView& t, int k, in
 #pragma unroll
   for (int i = MAXV - 1; i >= 3; --i)
     if (i < t.nmax && row[i] == kFill) len = i;
-  const double tol = pmax(kMinTolerance, kToleranceFactor * pmax(box.xmax - box.xmin, box.ymax - box.ymin));
+  double xmin = poly[0].x, xmax = poly[0].x, ymin = poly[0].y, ymax = poly[0].y;
+#pragma unroll
+  for (int i = 1; i < MAXV; ++i) {
+    xmin = pmin(xmin, poly[i].x);
+    xmax = pmax(xmax, poly[i].x);
+    ymin = pmin(ymin, poly[i].y);
+    ymax = pmax(ymax, poly[i].y);
+  }
+  const double tol = pmax(kMinTolerance, kToleranceFactor * pmax(xmax - xmin, ymax - ymin));
   return cyrus_beck<MAXV>(a, b, poly, len, tol, cc, dd);
 }
 
