@@ -17,10 +17,19 @@
 // subquery j in processing order):
 //   ct_split_queries (count)   parts[i] = number of subqueries of query order[i]
 //   ct_split_queries (write)   subs[part_offsets[i] + k] = {order[i], frontier node k}
-//   *_count_sub                counts[j] (+ hit record per subquery)
+// boxes:
+//   ct_locate_boxes_count_sub  counts[j] (+ hit record per subquery)
 //   ct_subquery_offsets        counts[j] -> the exclusive prefix within its query;
 //                              per_query[q] = the query's total (then ct_exclusive_scan)
-//   *_write_sub                rows of subquery j at query_offsets[q] + prefix[j]
+//   ct_locate_boxes_write_sub  rows of subquery j at query_offsets[q] + prefix[j]
+// segments (every hit is clipped once, in the count pass, and its piece kept):
+//   ct_locate_edges_count_sub  walk + candidate list, wave-cooperative clip, counts[j], the
+//                              hit cells and clipped pieces of each subquery
+//   (exclusive scan of counts) staging offsets sub_offsets[j]
+//   ct_subquery_spans          per_query[q] and each query's first subquery
+//   (exclusive scan)           query_start[q]
+//   ct_locate_edges_write_sub  rows (query, cell, piece, sort key t) into staging
+//   ct_sort_staged_rows        each query's rows ranked by t into the output
 #include "ct_internal.h"
 #include "ct_primitives.h"
 #include "ct_traverse.h"
