@@ -1,13 +1,14 @@
-// ct_clip.h -- Sutherland-Hodgman polygon clipping held entirely in registers, shared by the
-// face-pair kernels (ct_faces.hip) and the regrid forward / adjoint (ct_regrid.hip).
+// ct_clip.h -- Sutherland-Hodgman polygon clipping for the face-pair kernels (ct_faces.hip)
+// and the regrid forward (ct_regrid.hip).
 //
 // A per-lane array indexed by a runtime value is placed in scratch memory by the compiler;
 // for clipping (an output vertex is appended at a runtime position, an input vertex read at
 // a runtime position) that made the kernels bound by scratch traffic.  Here every loop over
-// a polygon is unrolled to its compile-time capacity with a `j < len` guard, and the two
-// runtime-indexed accesses go through put() / get(): one compare + select per slot, which
-// keeps the arrays in VGPRs.  The arithmetic and the branch structure are the reference's
-// (sutherland_hodgman.py:56-148), operation for operation.
+// a polygon is unrolled to its compile-time capacity with a `j < len` guard, so a stage reads
+// its input polygon from registers with static indices, and the runtime-indexed accesses (the
+// appended output vertices, the input's last vertex) go to the lane's row in LDS.  After each
+// stage the row is read back into registers.  The arithmetic and the branch structure are the
+// reference's (sutherland_hodgman.py:56-148), operation for operation.
 #pragma once
 
 #include "ct_internal.h"
@@ -15,18 +16,11 @@
 namespace ct {
 namespace {
 
-// arr[n] = v for a runtime n, as one select per slot (written as a conditional expression,
-// which compiles to v_cndmask rather than a branch per slot).
+// a ? x : y for points and ints, as selects (v_cndmask) rather than branches.
 __device__ __forceinline__ P2 sel(bool c, P2 a, P2 b) { return P2{c ? a.x : b.x, c ? a.y : b.y}; }
 __device__ __forceinline__ int sel(bool c, int a, int b) { return c ? a : b; }
 
-template <typename T, int CAP>
-__device__ __forceinline__ void put(T (&arr)[CAP], int n, T v) {
-#pragma unroll
-  for (int k = 0; k < CAP; ++k) arr[k] = sel(k == n, v, arr[k]);
-}
-
-// arr[n] for a runtime n in [0, CAP).
+// arr[n] for a runtime n in [0, CAP): one select per slot.
 template <typename T, int CAP>
 __device__ __forceinline__ T get(const T (&arr)[CAP], int n) {
   T v = arr[0];
@@ -35,24 +29,42 @@ __device__ __forceinline__ T get(const T (&arr)[CAP], int n) {
   return v;
 }
 
+// A lane's clip row in LDS: slot s at base[s * kBlock], lane-interleaved, so the lanes of a
+// wavefront touch 64 consecutive 16-byte words whatever slot each of them uses (no bank
+// conflicts).  A stage appends its output vertices here at runtime positions (one 16-byte
+// store each); the selects that would keep a runtime-indexed output in registers cost CAP
+// compares and 4 * CAP v_cndmask per appended vertex, and were most of the clip's
+// instructions.
+struct ClipRow {
+  double2* base;
+  __device__ __forceinline__ void put(int s, P2 v) const { base[s * kBlock] = double2{v.x, v.y}; }
+  __device__ __forceinline__ P2 get(int s) const {
+    const double2 d = base[s * kBlock];
+    return P2{d.x, d.y};
+  }
+};
+
+// Clip rows of capacity CAP for the kBlock lanes of a workgroup: CAP * 4 KiB of LDS.  Every
+// lane touches only its own slots, so no barrier is needed.
+#define CT_CLIP_ROWS(CAP, name)                       \
+  __shared__ double2 name##_lds[(CAP) * kBlock];      \
+  const ClipRow name { name##_lds + threadIdx.x }
+
 // sutherland_hodgman.py:56-60 (strict)
 __device__ __forceinline__ bool sh_in(P2 p, P2 r, double ux, double uy) {
   return ux * (p.y - r.y) > uy * (p.x - r.x);
 }
 
-// One clip stage (sutherland_hodgman.py:96-145): polygon `in` (len vertices) against the clip
-// edge r -> s, into `out`; returns the output length.  With PROV, src[o] records where output
-// vertex o came from: j for a copy of in[j], -(1 + j) for the intersection of input edge
-// (aprev[o] -> j) with the clip line.  Writes past CAP are dropped (a convex n-gon clipped by
-// a convex m-gon has at most n + m vertices).
-template <int CAP, bool PROV>
-__device__ __forceinline__ int sh_stage(const P2 (&in)[CAP], int len, P2 r, P2 s, P2 (&out)[CAP],
-                                        int (&src)[CAP], int (&aprev)[CAP]) {
+// One clip stage (sutherland_hodgman.py:96-145): polygon `in` (len vertices, also held in
+// `row` slots [0, len)) against the clip edge r -> s; the output polygon is written to `row`
+// (over the input, which is read from registers) and its length returned.  Writes past CAP
+// are dropped (a convex n-gon clipped by a convex m-gon has at most n + m vertices).
+template <int CAP>
+__device__ __forceinline__ int sh_stage(const P2 (&in)[CAP], int len, P2 r, P2 s, const ClipRow& row) {
   const double ux = s.x - r.x, uy = s.y - r.y;
   const double nx = -uy, ny = ux;
   int n_out = 0;
-  P2 a = get(in, len - 1);
-  int ai = len - 1;
+  P2 a = row.get(len - 1);
   bool a_in = sh_in(a, r, ux, uy);
 #pragma unroll
   for (int j = 0; j < CAP; ++j) {
@@ -77,23 +89,14 @@ __device__ __forceinline__ int sh_stage(const P2 (&in)[CAP], int len, P2 r, P2 s
         if (cross_pt && n_out < CAP) {
           const double nw = nx * (r.x - a.x) + ny * (r.y - a.y);
           const double t = nw / nv;
-          put(out, n_out, P2{a.x + t * vx, a.y + t * vy});
-          if (PROV) {
-            put(src, n_out, -(1 + j));
-            put(aprev, n_out, ai);
-          }
+          row.put(n_out, P2{a.x + t * vx, a.y + t * vy});
           ++n_out;
         }
         if (copy_b && n_out < CAP) {
-          put(out, n_out, b);
-          if (PROV) {
-            put(src, n_out, j);
-            put(aprev, n_out, ai);
-          }
+          row.put(n_out, b);
           ++n_out;
         }
         a = b;
-        ai = j;
         a_in = b_in;
       }
     }
@@ -103,23 +106,24 @@ __device__ __forceinline__ int sh_stage(const P2 (&in)[CAP], int len, P2 r, P2 s
 
 // Stages 0 .. upto-1 of clipping `subj` (ls vertices) by `clip` (lc vertices), skipping
 // zero-length clip edges (sutherland_hodgman.py:105-107); returns the polygon entering stage
-// `upto`, or a length < 3 as soon as a stage leaves fewer than 3 vertices (area 0).
+// `upto` in `poly`, or a length < 3 as soon as a stage leaves fewer than 3 vertices (area 0).
 template <int MA, int MB, int CAP>
 __device__ __forceinline__ int sh_run(const P2 (&subj)[MA], int ls, const P2 (&clip)[MB], int lc, int upto,
-                                      P2 (&poly)[CAP]) {
+                                      const ClipRow& row, P2 (&poly)[CAP]) {
 #pragma unroll
   for (int i = 0; i < CAP; ++i) poly[i] = (i < MA) ? subj[i < MA ? i : 0] : P2{0.0, 0.0};
+#pragma unroll
+  for (int i = 0; i < MA; ++i) row.put(i, subj[i]);
   int len = ls;
   P2 r = get(clip, lc - 1);
   for (int ci = 0; ci < upto; ++ci) {
     const P2 s = get(clip, ci);
     if (!(s.x - r.x == 0 && s.y - r.y == 0)) {
-      P2 tmp[CAP];
-      int src[CAP], aprev[CAP];
-      len = sh_stage<CAP, false>(poly, len, r, s, tmp, src, aprev);
-#pragma unroll
-      for (int i = 0; i < CAP; ++i) poly[i] = tmp[i];
+      len = sh_stage<CAP>(poly, len, r, s, row);
       if (len < 3) return len;
+#pragma unroll
+      for (int i = 0; i < CAP; ++i)
+        if (i < len) poly[i] = row.get(i);
     }
     r = s;
   }
@@ -180,15 +184,23 @@ __device__ __forceinline__ double clip_area_indexed(const P2 (&subj)[MA], int ls
   return polygon_area<CAP>(outp, n_out);
 }
 
-// sutherland_hodgman.py:84-148 polygon_polygon_clip_area.
+// sutherland_hodgman.py:84-148 polygon_polygon_clip_area.  `row` is the lane's LDS clip row
+// of clip_row_cap<MA, MB>() slots (CT_CLIP_ROWS); the indexed path for the large classes does
+// not use it.
 template <int MA, int MB>
-__device__ __forceinline__ double clip_area(const P2 (&subj)[MA], int ls, const P2 (&clip)[MB], int lc) {
+constexpr int clip_row_cap() {
+  return MA + MB > 16 ? 1 : MA + MB;
+}
+
+template <int MA, int MB>
+__device__ __forceinline__ double clip_area(const P2 (&subj)[MA], int ls, const P2 (&clip)[MB], int lc,
+                                            const ClipRow& row) {
   constexpr int CAP = MA + MB;
   if constexpr (CAP > 16) {
     return clip_area_indexed<MA, MB>(subj, ls, clip, lc);
   } else {
     P2 poly[CAP];
-    const int len = sh_run<MA, MB, CAP>(subj, ls, clip, lc, lc, poly);
+    const int len = sh_run<MA, MB, CAP>(subj, ls, clip, lc, lc, row, poly);
     return (len < 3) ? 0.0 : polygon_area<CAP>(poly, len);
   }
 }

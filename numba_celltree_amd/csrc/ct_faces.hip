@@ -79,8 +79,9 @@ __global__ void __launch_bounds__(kBlock) k_face_pairs(const double* __restrict_
     keep[k] = sat ? 1 : 0;
     return;
   }
+  CT_CLIP_ROWS((clip_row_cap<MA, MB>()), row);
   double ar = 0.0;
-  if (sat) ar = clip_area<MA, MB>(a, la, b, lb);
+  if (sat) ar = clip_area<MA, MB>(a, la, b, lb, row);
   area[k] = ar;
   keep[k] = (sat && ar > 0) ? 1 : 0;
 }
@@ -98,7 +99,8 @@ __global__ void __launch_bounds__(kBlock) k_area(const double* __restrict__ va, 
   P2 a[MA], b[MB];
   const int la = load_face<MA>(va, fa, na, ia[k], a);
   const int lb = load_face<MB>(vb, fb, nbm, ib[k], b);
-  out[k] = clip_area<MA, MB>(a, la, b, lb);
+  CT_CLIP_ROWS((clip_row_cap<MA, MB>()), row);
+  out[k] = clip_area<MA, MB>(a, la, b, lb, row);
 }
 
 // separating_axis.py:58-75 polygons_intersect.
@@ -133,7 +135,8 @@ __global__ void __launch_bounds__(kBlock) k_box_pairs(const double* __restrict__
   const P2 a[4] = {{bx.xmin, bx.ymin}, {bx.xmax, bx.ymin}, {bx.xmax, bx.ymax}, {bx.xmin, bx.ymax}};
   P2 b[MB];
   const int lb = load_face<MB>(vb, fb, nbm, jj[k], b);
-  const double ar = clip_area<4, MB>(a, 4, b, lb);
+  CT_CLIP_ROWS((clip_row_cap<4, MB>()), row);
+  const double ar = clip_area<4, MB>(a, 4, b, lb, row);
   area[k] = ar;
   if (keep) keep[k] = ar > 0 ? 1 : 0;
 }

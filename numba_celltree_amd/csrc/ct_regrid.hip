@@ -15,8 +15,9 @@
 // atomics, in a fixed order (per-pair corner rows, per-face corner sums, per-vertex CSR
 // gathers), so they are identical from run to run.
 //
-// Every polygon lives in registers (ct_clip.h): loops run to the compile-time capacity with
-// a length guard and runtime-indexed accesses go through select chains, so no kernel here
+// Input polygons live in registers: loops run to the compile-time capacity with a length
+// guard.  Runtime-indexed data goes to per-lane rows in LDS (the clip's output polygon,
+// ct_clip.h; the adjoint's target-corner sums) or through select chains, so no kernel here
 // touches scratch memory.
 #include "ct_clip.h"
 #include "ct_internal.h"
@@ -111,7 +112,8 @@ __global__ void __launch_bounds__(kBlock) k_pair_areas(const double* __restrict_
   int ai[MA], bi[MB];
   const int la = load_face_idx<MA>(tf, ntm, tt[k], tv, a, ai);
   const int lb = load_face_idx<MB>(sf, nsm, ss[k], sv, b, bi);
-  area[k] = clip_area<MA, MB>(a, la, b, lb);
+  CT_CLIP_ROWS((clip_row_cap<MA, MB>()), row);
+  area[k] = clip_area<MA, MB>(a, la, b, lb, row);
 }
 
 // Per pair: gA = sum_f gy[t,f] F[s,f] / area_t, times the derivative of the overlap area.
@@ -179,9 +181,21 @@ __device__ __forceinline__ void edge_range_gradient(double alpha, double beta, d
   g1 = {gA * w1 * dy, -gA * w1 * dx};
 }
 
+// A lane's target-corner accumulators in LDS (slot i at base[i * kBlock]), so the 2 * MA
+// registers they would take stay free for occupancy.
+struct CornerRow {
+  double2* base;
+  __device__ __forceinline__ void add(int i, double x, double y) const {
+    double2 v = base[i * kBlock];
+    v.x += x;
+    v.y += y;
+    base[i * kBlock] = v;
+  }
+};
+
 template <int MA, int MB>
 __device__ __forceinline__ void overlap_area_gradient(const P2 (&a)[MA], int la, const P2 (&b)[MB], int lb,
-                                                      double gA, P2 (&ga)[MA], P2 (&gb)[MB]) {
+                                                      double gA, const CornerRow& ga, P2 (&gb)[MB]) {
   const P2 b_last = get(b, lb - 1);
   // target edges a[i] -> a[i + 1 mod la] against the source's open half-planes
 #pragma unroll
@@ -202,15 +216,10 @@ __device__ __forceinline__ void overlap_area_gradient(const P2 (&a)[MA], int la,
       if (!e.empty && t1 > t0) {
         P2 g0, g1;
         edge_range_gradient(t0, t1, dx, dy, gA, g0, g1);
-        ga[i].x += g0.x;
-        ga[i].y += g0.y;
+        ga.add(i, g0.x, g0.y);
         const bool wrap = !(i + 1 < la);
-        if (i + 1 < MA) {
-          ga[(i + 1 < MA) ? i + 1 : 0].x += wrap ? 0.0 : g1.x;
-          ga[(i + 1 < MA) ? i + 1 : 0].y += wrap ? 0.0 : g1.y;
-        }
-        ga[0].x += wrap ? g1.x : 0.0;
-        ga[0].y += wrap ? g1.y : 0.0;
+        if (i + 1 < MA) ga.add((i + 1 < MA) ? i + 1 : 0, wrap ? 0.0 : g1.x, wrap ? 0.0 : g1.y);
+        ga.add(0, wrap ? g1.x : 0.0, wrap ? g1.y : 0.0);
       }
     }
   }
@@ -249,80 +258,36 @@ __device__ __forceinline__ void overlap_area_gradient(const P2 (&a)[MA], int la,
 }
 
 // Vertex gradients without atomics, in a fixed summation order (deterministic):
-//   k_pair_backward       pair k -> GA[k] (its target's corners), GB[k] (its source's corners)
-//   k_target_corners      target t -> TC[t] = sum of GA over t's pairs (pair order), plus the
-//                         target-area term -sum_f gy y / area_t through the fan area
-//   k_source_corners      source s -> SC[s] = sum of GB over s's pairs (by-source order)
-//   k_vertex_gather       vertex v -> sum of the corner rows of v, in corner order (CSR)
+//   k_pair_backward         pair k -> GB[k] (its source's corners) and its target's corners: a
+//                           target whose pairs all lie in the workgroup's block of kBlock
+//                           consecutive pairs is summed there (TC[t], from the rows in LDS, in
+//                           pair order, plus the target-area term); the pairs of the others
+//                           write GA[k]
+//   k_target_corners        target t -> TC[t] = sum of GA over t's pairs (pair order), plus the
+//                           target-area term -sum_f gy y / area_t through the fan area, for the
+//                           targets k_pair_backward did not finish
+//   k_source_corners        source s -> SC[s] = sum of GB over s's pairs (by-source order)
+//   k_vertex_gather         vertex v -> sum of the corner rows (TC or SC) of v, in corner order
 // Corner rows have one P2 per face slot (stride n_max of that mesh).
-template <int MA, int MB>
-__global__ void __launch_bounds__(kBlock) k_pair_backward(
-    const double* __restrict__ tv, const int32_t* __restrict__ tf, int ntm, const double* __restrict__ sv,
-    const int32_t* __restrict__ sf, int nsm, const int64_t* __restrict__ tt, const int64_t* __restrict__ ss,
-    int64_t n, const double* __restrict__ F, const double* __restrict__ gy, const double* __restrict__ tarea,
-    int nf, double* __restrict__ GA, double* __restrict__ GB) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const int64_t t = tt[k], s = ss[k];
-  double gA = 0.0;
-  const double* gyt = gy + t * nf;
-  const double* Fs = F + s * nf;
-  int f = 0;
-  for (; f + 4 <= nf; f += 4) {  // four fields in flight, summed in field order
-    const double y0 = gyt[f], y1 = gyt[f + 1], y2 = gyt[f + 2], y3 = gyt[f + 3];
-    const double f0 = Fs[f], f1 = Fs[f + 1], f2 = Fs[f + 2], f3 = Fs[f + 3];
-    gA += y0 * f0;
-    gA += y1 * f1;
-    gA += y2 * f2;
-    gA += y3 * f3;
-  }
-  for (; f < nf; ++f) gA += gyt[f] * Fs[f];
-  gA /= tarea[t];
-  P2 ga[MA], gb[MB];
-#pragma unroll
-  for (int i = 0; i < MA; ++i) ga[i] = {0.0, 0.0};
-#pragma unroll
-  for (int i = 0; i < MB; ++i) gb[i] = {0.0, 0.0};
-  if (gA != 0.0) {
-    P2 a[MA], b[MB];
-    int ai[MA], bi[MB];
-    const int la = load_face_idx<MA>(tf, ntm, t, tv, a, ai);
-    const int lb = load_face_idx<MB>(sf, nsm, s, sv, b, bi);
-    overlap_area_gradient<MA, MB>(a, la, b, lb, gA, ga, gb);
-  }
-  double2* ra = reinterpret_cast<double2*>(GA) + k * ntm;
-  double2* rb = reinterpret_cast<double2*>(GB) + k * nsm;
-#pragma unroll
-  for (int i = 0; i < MA; ++i)
-    if (i < ntm) ra[i] = make_double2(ga[i].x, ga[i].y);
-#pragma unroll
-  for (int i = 0; i < MB; ++i)
-    if (i < nsm) rb[i] = make_double2(gb[i].x, gb[i].y);
+
+// True when target pairs [begin, end) all fall in one block of kBlock consecutive pairs, the
+// pairs of one k_pair_backward workgroup.
+__device__ __forceinline__ bool whole_in_block(int64_t begin, int64_t end) {
+  return end > begin && begin / kBlock == (end - 1) / kBlock;
 }
 
-// Target corners, one lane per corner (t, i) = c / nmax, c % nmax, so the lanes of a target
-// read each pair row GA[k] together: the pairs of target t are toff[t] .. toff[t + 1]
-// (intersect_faces order).  The target-area term is the fan-area adjoint, evaluated by every
-// lane of the target and kept for its own slot.
-template <int MAXV>
-__global__ void __launch_bounds__(kBlock) k_target_corners(const double* __restrict__ v, const int32_t* __restrict__ f,
-                                                           int nmax, int64_t n, const int64_t* __restrict__ toff,
-                                                           const double* __restrict__ GA,
-                                                           const double* __restrict__ y,
-                                                           const double* __restrict__ gy,
-                                                           const double* __restrict__ tarea, int nf,
-                                                           double* __restrict__ TC) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n * nmax) return;
-  const int64_t t = c / nmax;
-  const int slot = (int)(c - t * nmax);
+// TC[t] slot `slot` = (sum over the target's pairs of `row(k)`, in pair order) plus the
+// target-area term of that slot, as k_target_corners sums it.
+template <int MAXV, typename Row>
+__device__ __forceinline__ P2 target_corner(const double* __restrict__ v, const int32_t* __restrict__ f,
+                                            int nmax, int64_t t, int slot, int64_t begin, int64_t end,
+                                            Row row, const double* __restrict__ y,
+                                            const double* __restrict__ gy, const double* __restrict__ tarea,
+                                            int nf) {
   double gx = 0.0, gyv = 0.0;
-  const double2* ga = reinterpret_cast<const double2*>(GA) + slot;
-  // four rows in flight, summed in pair order
-  int64_t k = toff[t];
-  const int64_t end = toff[t + 1];
-  for (; k + 4 <= end; k += 4) {
-    const double2 d0 = ga[k * nmax], d1 = ga[(k + 1) * nmax], d2 = ga[(k + 2) * nmax], d3 = ga[(k + 3) * nmax];
+  int64_t k = begin;
+  for (; k + 4 <= end; k += 4) {  // four rows in flight, summed in pair order
+    const double2 d0 = row(k), d1 = row(k + 1), d2 = row(k + 2), d3 = row(k + 3);
     gx += d0.x;
     gyv += d0.y;
     gx += d1.x;
@@ -333,7 +298,7 @@ __global__ void __launch_bounds__(kBlock) k_target_corners(const double* __restr
     gyv += d3.y;
   }
   for (; k < end; ++k) {
-    const double2 d = ga[k * nmax];
+    const double2 d = row(k);
     gx += d.x;
     gyv += d.y;
   }
@@ -363,7 +328,105 @@ __global__ void __launch_bounds__(kBlock) k_target_corners(const double* __restr
     gx += mine.x;
     gyv += mine.y;
   }
-  reinterpret_cast<double2*>(TC)[c] = make_double2(gx, gyv);
+  return P2{gx, gyv};
+}
+
+// One lane per pair k = blockIdx.x * kBlock + lane.
+template <int MA, int MB>
+__global__ void __launch_bounds__(kBlock) k_pair_backward(
+    const double* __restrict__ tv, const int32_t* __restrict__ tf, int ntm, const double* __restrict__ sv,
+    const int32_t* __restrict__ sf, int nsm, const int64_t* __restrict__ tt, const int64_t* __restrict__ ss,
+    int64_t n, const double* __restrict__ F, const double* __restrict__ gy, const double* __restrict__ tarea,
+    int nf, const int64_t* __restrict__ toff, const double* __restrict__ y,
+    double* __restrict__ GA, double* __restrict__ GB, double* __restrict__ TC) {
+  extern __shared__ double2 rows[];  // ntm slots per lane: slot i of lane l at rows[i * kBlock + l]
+  __shared__ int starts[kBlock];     // lanes where a target finished here begins
+  __shared__ int n_starts;
+  const int lane = threadIdx.x;
+  const int64_t k0 = (int64_t)blockIdx.x * kBlock;
+  const int64_t k = k0 + lane;
+  if (lane == 0) n_starts = 0;
+  __syncthreads();
+  if (k < n) {
+    const int64_t t = tt[k], s = ss[k];
+    double gA = 0.0;
+    const double* gyt = gy + t * nf;
+    const double* Fs = F + s * nf;
+    int f = 0;
+    for (; f + 4 <= nf; f += 4) {  // four fields in flight, summed in field order
+      const double y0 = gyt[f], y1 = gyt[f + 1], y2 = gyt[f + 2], y3 = gyt[f + 3];
+      const double f0 = Fs[f], f1 = Fs[f + 1], f2 = Fs[f + 2], f3 = Fs[f + 3];
+      gA += y0 * f0;
+      gA += y1 * f1;
+      gA += y2 * f2;
+      gA += y3 * f3;
+    }
+    for (; f < nf; ++f) gA += gyt[f] * Fs[f];
+    gA /= tarea[t];
+    const CornerRow ga{rows + lane};
+    for (int i = 0; i < ntm; ++i) rows[i * kBlock + lane] = make_double2(0.0, 0.0);
+    P2 gb[MB];
+#pragma unroll
+    for (int i = 0; i < MB; ++i) gb[i] = {0.0, 0.0};
+    if (gA != 0.0) {
+      P2 a[MA], b[MB];
+      int ai[MA], bi[MB];
+      const int la = load_face_idx<MA>(tf, ntm, t, tv, a, ai);
+      const int lb = load_face_idx<MB>(sf, nsm, s, sv, b, bi);
+      overlap_area_gradient<MA, MB>(a, la, b, lb, gA, ga, gb);
+    }
+    double2* rb = reinterpret_cast<double2*>(GB) + k * nsm;
+#pragma unroll
+    for (int i = 0; i < MB; ++i)
+      if (i < nsm) rb[i] = make_double2(gb[i].x, gb[i].y);
+    const int64_t begin = toff[t], end = toff[t + 1];
+    if (whole_in_block(begin, end)) {
+      if (begin == k) starts[atomicAdd(&n_starts, 1)] = lane;
+    } else {
+      double2* ra = reinterpret_cast<double2*>(GA) + k * ntm;
+      for (int i = 0; i < ntm; ++i) ra[i] = rows[i * kBlock + lane];
+    }
+  }
+  __syncthreads();
+  // The corners of the targets finished here, one lane per (target, slot); the summation
+  // order of a target does not depend on which lane takes it.
+  const int work = n_starts * ntm;
+  for (int w = lane; w < work; w += kBlock) {
+    const int q = w / ntm;
+    const int slot = w - q * ntm;
+    const int first = starts[q];
+    const int64_t t = tt[k0 + first];
+    const int64_t end = toff[t + 1];
+    const P2 c = target_corner<MA>(tv, tf, ntm, t, slot, k0 + first, end,
+                                   [&](int64_t kk) { return rows[slot * kBlock + (int)(kk - k0)]; }, y, gy,
+                                   tarea, nf);
+    reinterpret_cast<double2*>(TC)[t * ntm + slot] = make_double2(c.x, c.y);
+  }
+}
+
+// Target corners, one lane per corner (t, i) = c / nmax, c % nmax, so the lanes of a target
+// read each pair row GA[k] together: the pairs of target t are toff[t] .. toff[t + 1]
+// (intersect_faces order).  The target-area term is the fan-area adjoint, evaluated by every
+// lane of the target and kept for its own slot.  Targets whose pairs share one block of
+// k_pair_backward were finished there and are skipped.
+template <int MAXV>
+__global__ void __launch_bounds__(kBlock) k_target_corners(const double* __restrict__ v, const int32_t* __restrict__ f,
+                                                           int nmax, int64_t n, const int64_t* __restrict__ toff,
+                                                           const double* __restrict__ GA,
+                                                           const double* __restrict__ y,
+                                                           const double* __restrict__ gy,
+                                                           const double* __restrict__ tarea, int nf,
+                                                           double* __restrict__ TC) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n * nmax) return;
+  const int64_t t = c / nmax;
+  const int slot = (int)(c - t * nmax);
+  const int64_t begin = toff[t], end = toff[t + 1];
+  if (whole_in_block(begin, end)) return;
+  const double2* ga = reinterpret_cast<const double2*>(GA) + slot;
+  const P2 r = target_corner<MAXV>(v, f, nmax, t, slot, begin, end,
+                                   [&](int64_t k) { return ga[k * nmax]; }, y, gy, tarea, nf);
+  reinterpret_cast<double2*>(TC)[c] = make_double2(r.x, r.y);
 }
 
 // Source corners, one lane per corner: the pairs of source s are perm[soff[s] .. soff[s + 1]).
@@ -465,34 +528,39 @@ __global__ void __launch_bounds__(kBlock) k_apply(const int64_t* __restrict__ to
   y[i] = acc / tarea[t];
 }
 
-// k_apply_t with one lane per source and all NF fields in registers: each pair's weight and
-// target are loaded once and its gy row is read as a whole, instead of by NF lanes that each
-// repeat the index and weight loads.  Same sums, same order, so the same bits.
-template <int NF>
+// k_apply_t with nf / FPL lanes per source, each holding FPL consecutive fields in registers:
+// a pair's weight and target are loaded by the source's lanes together (one request) and its
+// gy row is read as nf / FPL contiguous pieces, instead of by nf lanes that each repeat the
+// index and weight loads, or by one lane whose 16 accumulators leave few wavefronts to hide
+// the gathers (16 fields, 1M sources: 163 -> 137 us).  Same sums, same order, so the same bits.
+template <int FPL>
 __global__ void __launch_bounds__(kBlock) k_apply_t_rows(const int64_t* __restrict__ soff,
                                                          const int64_t* __restrict__ perm,
                                                          const int64_t* __restrict__ src_tgt,
                                                          const double* __restrict__ area,
                                                          const double* __restrict__ tarea,
-                                                         const double* __restrict__ gy, int64_t n_s,
+                                                         const double* __restrict__ gy, int nf, int64_t n_s,
                                                          double* __restrict__ gF) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n_s) return;
-  double acc[NF];
+  const int parts = nf / FPL;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_s * parts) return;
+  const int64_t s = i / parts;
+  const int f0 = (int)(i - s * parts) * FPL;
+  double acc[FPL];
 #pragma unroll
-  for (int f = 0; f < NF; ++f) acc[f] = 0.0;
+  for (int f = 0; f < FPL; ++f) acc[f] = 0.0;
   const int64_t end = soff[s + 1];
   for (int64_t m = soff[s]; m < end; ++m) {
     const int64_t k = perm[m];
     const int64_t t = src_tgt[m];
     const double w = area[k] / tarea[t];
-    const double* row = gy + t * NF;
+    const double* row = gy + t * nf + f0;
 #pragma unroll
-    for (int f = 0; f < NF; ++f) acc[f] += w * row[f];
+    for (int f = 0; f < FPL; ++f) acc[f] += w * row[f];
   }
-  double* out = gF + s * NF;
+  double* out = gF + s * nf + f0;
 #pragma unroll
-  for (int f = 0; f < NF; ++f) out[f] = acc[f];
+  for (int f = 0; f < FPL; ++f) out[f] = acc[f];
 }
 
 // gF[s, f] = sum over the source's pairs (perm order) of area_k / area_t gy[t_k, f].
@@ -798,6 +866,22 @@ inline int vclass(int nmax) { return nmax <= 4 ? 4 : nmax <= 8 ? 8 : 16; }
     }                                                                                        \
   } while (0)
 
+// gF by the k_apply_t_rows form that fits n_fields.
+int launch_apply_t(const int64_t* so, const int64_t* pm, const int64_t* tg, const double* area,
+                   const double* tgt_area, const double* gy, int32_t n_fields, int64_t n_src, double* gF,
+                   hipStream_t st) {
+  if (n_fields % 4 == 0)
+    k_apply_t_rows<4><<<grid_for(n_src * (n_fields / 4)), kBlock, 0, st>>>(so, pm, tg, area, tgt_area, gy, n_fields,
+                                                                          n_src, gF);
+  else if (n_fields % 2 == 0)
+    k_apply_t_rows<2><<<grid_for(n_src * (n_fields / 2)), kBlock, 0, st>>>(so, pm, tg, area, tgt_area, gy, n_fields,
+                                                                          n_src, gF);
+  else
+    k_apply_t<<<grid_for(n_src * n_fields), kBlock, 0, st>>>(so, pm, tg, area, tgt_area, gy, n_fields, n_src, gF);
+  CT_LAUNCH_CHECK();
+  return CT_OK;
+}
+
 }  // namespace
 }  // namespace ct
 
@@ -844,22 +928,7 @@ int ct_regrid_apply_transpose(const int64_t* src_offsets, const int64_t* perm, c
                               int64_t n_src, double* gF, void* stream) {
   CT_REQUIRE(n_src >= 0 && n_fields >= 1, "bad sizes");
   if (n_src == 0) return CT_OK;
-  hipStream_t st = as_stream(stream);
-  const unsigned g = grid_for(n_src);
-  switch (n_fields) {
-    case 1: k_apply_t_rows<1><<<g, kBlock, 0, st>>>(src_offsets, perm, src_tgt, area, tgt_area, gy, n_src, gF); break;
-    case 2: k_apply_t_rows<2><<<g, kBlock, 0, st>>>(src_offsets, perm, src_tgt, area, tgt_area, gy, n_src, gF); break;
-    case 4: k_apply_t_rows<4><<<g, kBlock, 0, st>>>(src_offsets, perm, src_tgt, area, tgt_area, gy, n_src, gF); break;
-    case 8: k_apply_t_rows<8><<<g, kBlock, 0, st>>>(src_offsets, perm, src_tgt, area, tgt_area, gy, n_src, gF); break;
-    case 16:
-      k_apply_t_rows<16><<<g, kBlock, 0, st>>>(src_offsets, perm, src_tgt, area, tgt_area, gy, n_src, gF);
-      break;
-    default:
-      k_apply_t<<<grid_for(n_src * n_fields), kBlock, 0, st>>>(src_offsets, perm, src_tgt, area, tgt_area, gy,
-                                                               n_fields, n_src, gF);
-  }
-  CT_LAUNCH_CHECK();
-  return CT_OK;
+  return launch_apply_t(src_offsets, perm, src_tgt, area, tgt_area, gy, n_fields, n_src, gF, as_stream(stream));
 }
 
 int ct_barycentric_triangle_backward(const double* points, const int64_t* face_indices, int64_t n,
@@ -930,8 +999,17 @@ int ct_regrid_vertex_backward(const ct_regrid_topology* topo, const double* tgt_
   const int64_t n_pairs = topo->n_pairs, n_tgt = topo->n_tgt, n_src = topo->n_src;
   if (n_pairs > 0) {
     const unsigned g = grid_for(n_pairs);
-    CT_PAIR_DISPATCH(k_pair_backward, n_max_t, n_max_s, tgt_vertices, topo->tgt_faces, n_max_t, src_vertices,
-                     topo->src_faces, n_max_s, topo->tt, topo->ss, n_pairs, F, gy, tgt_area, n_fields, GA, GB);
+    const size_t lds = sizeof(double2) * kBlock * n_max_t;  // the per-lane corner rows
+#define CT_PAIR_BACKWARD(X, Y)                                                                                  \
+  k_pair_backward<X, Y><<<g, kBlock, lds, st>>>(tgt_vertices, topo->tgt_faces, n_max_t, src_vertices,           \
+                                                topo->src_faces, n_max_s, topo->tt, topo->ss, n_pairs, F, gy,   \
+                                                tgt_area, n_fields, topo->tgt_offsets, y, GA, GB, TC)
+    const int ca = vclass(n_max_t), cb = vclass(n_max_s);
+    if (ca == 4 && cb == 4) CT_PAIR_BACKWARD(4, 4);
+    else if (ca == 8 && cb == 4) CT_PAIR_BACKWARD(8, 4);
+    else if (ca == 4 && cb == 8) CT_PAIR_BACKWARD(4, 8);
+    else CT_PAIR_BACKWARD(8, 8);
+#undef CT_PAIR_BACKWARD
     CT_LAUNCH_CHECK();
   }
   if (n_tgt > 0) {
