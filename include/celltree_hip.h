@@ -476,7 +476,17 @@ typedef struct ct_regrid_topology {
   int64_t n_src_vertices;
   int32_t n_max_t;                   /* 3 .. 8 */
   int32_t n_max_s;                   /* 3 .. 8 */
+  const int64_t* rest_targets;       /* optional (may be NULL): the targets whose corner sums the
+                                        vertex backward's pair pass does not finish (see
+                                        ct_regrid_rest_targets); NULL: every target is checked */
+  int64_t n_rest_targets;
 } ct_regrid_topology;
+
+/* Fills rest_targets (device, capacity n_tgt) with the targets the vertex backward's pair pass
+ * leaves to its per-target pass -- those without pairs and those whose pairs cross a block of
+ * the pair pass -- and *n_out (HOST) with their count.  Depends on the topology only, so it
+ * is called once per topology.  Synchronizes the stream. */
+int ct_regrid_rest_targets(const ct_regrid_topology* topo, int64_t* rest_targets, int64_t* n_out, void* stream);
 
 size_t ct_regrid_backward_workspace_bytes(const ct_regrid_topology* topo);
 

@@ -110,6 +110,8 @@ class CtRegridTopology(ctypes.Structure):
         ("n_src_vertices", ctypes.c_int64),
         ("n_max_t", ctypes.c_int32),
         ("n_max_s", ctypes.c_int32),
+        ("rest_targets", ctypes.c_void_p),
+        ("n_rest_targets", ctypes.c_int64),
     ]
 
 
@@ -192,6 +194,7 @@ SIGNATURES = {
     "ct_regrid_apply": (ctypes.c_int, [_P, _P, _P, _P, _P, _I32, _I64, _P, _P]),
     "ct_regrid_apply_transpose": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _I32, _I64, _P, _P]),
     "ct_barycentric_triangle_backward": (ctypes.c_int, [_P, _P, _I64, _P, _I32, _P, _P, _P, _P, _P]),
+    "ct_regrid_rest_targets": (ctypes.c_int, [_RP, _P, _P, _P]),
     "ct_regrid_backward_workspace_bytes": (_SZ, [_RP]),
     "ct_regrid_vertex_backward": (ctypes.c_int, [_RP, _P, _P, _P, _P, _P, _P, _I32, _P, _SZ, _P, _P, _P]),
     "ct_batch_locate_points": (ctypes.c_int, [_BP, _P, _P, _P]),

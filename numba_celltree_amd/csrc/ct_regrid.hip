@@ -19,6 +19,8 @@
 // guard.  Runtime-indexed data goes to per-lane rows in LDS (the clip's output polygon,
 // ct_clip.h; the adjoint's target-corner sums) or through select chains, so no kernel here
 // touches scratch memory.
+#include <vector>
+
 #include "ct_clip.h"
 #include "ct_internal.h"
 #include "ct_primitives.h"
@@ -268,11 +270,12 @@ __device__ __forceinline__ void overlap_area_gradient(const P2 (&a)[MA], int la,
 //                           targets k_pair_backward did not finish
 //   k_source_corners        source s -> SC[s] = sum of GB over s's pairs (by-source order)
 //   k_vertex_gather         vertex v -> sum of the corner rows (TC or SC) of v, in corner order
+//                           (both meshes in one launch)
 // Corner rows have one P2 per face slot (stride n_max of that mesh).
 
 // True when target pairs [begin, end) all fall in one block of kBlock consecutive pairs, the
 // pairs of one k_pair_backward workgroup.
-__device__ __forceinline__ bool whole_in_block(int64_t begin, int64_t end) {
+__host__ __device__ __forceinline__ bool whole_in_block(int64_t begin, int64_t end) {
   return end > begin && begin / kBlock == (end - 1) / kBlock;
 }
 
@@ -404,14 +407,16 @@ __global__ void __launch_bounds__(kBlock) k_pair_backward(
   }
 }
 
-// Target corners, one lane per corner (t, i) = c / nmax, c % nmax, so the lanes of a target
-// read each pair row GA[k] together: the pairs of target t are toff[t] .. toff[t + 1]
-// (intersect_faces order).  The target-area term is the fan-area adjoint, evaluated by every
-// lane of the target and kept for its own slot.  Targets whose pairs share one block of
-// k_pair_backward were finished there and are skipped.
+// Target corners, one lane per corner (t, i), so the lanes of a target read each pair row
+// GA[k] together: the pairs of target t are toff[t] .. toff[t + 1] (intersect_faces order).
+// The target-area term is the fan-area adjoint, evaluated by every lane of the target and kept
+// for its own slot.  Targets are rest[c / nmax] when the topology lists the targets the pair
+// kernel leaves (ct_regrid_rest_targets); otherwise c / nmax, skipping those whose pairs share
+// one block of k_pair_backward (finished there).
 template <int MAXV>
 __global__ void __launch_bounds__(kBlock) k_target_corners(const double* __restrict__ v, const int32_t* __restrict__ f,
-                                                           int nmax, int64_t n, const int64_t* __restrict__ toff,
+                                                           int nmax, int64_t n, const int64_t* __restrict__ rest,
+                                                           const int64_t* __restrict__ toff,
                                                            const double* __restrict__ GA,
                                                            const double* __restrict__ y,
                                                            const double* __restrict__ gy,
@@ -419,17 +424,21 @@ __global__ void __launch_bounds__(kBlock) k_target_corners(const double* __restr
                                                            double* __restrict__ TC) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n * nmax) return;
-  const int64_t t = c / nmax;
-  const int slot = (int)(c - t * nmax);
+  const int64_t q = c / nmax;
+  const int slot = (int)(c - q * nmax);
+  const int64_t t = rest ? rest[q] : q;
   const int64_t begin = toff[t], end = toff[t + 1];
-  if (whole_in_block(begin, end)) return;
+  if (!rest && whole_in_block(begin, end)) return;
   const double2* ga = reinterpret_cast<const double2*>(GA) + slot;
   const P2 r = target_corner<MAXV>(v, f, nmax, t, slot, begin, end,
                                    [&](int64_t k) { return ga[k * nmax]; }, y, gy, tarea, nf);
-  reinterpret_cast<double2*>(TC)[c] = make_double2(r.x, r.y);
+  reinterpret_cast<double2*>(TC)[t * nmax + slot] = make_double2(r.x, r.y);
 }
 
-// Source corners, one lane per corner: the pairs of source s are perm[soff[s] .. soff[s + 1]).
+// Source corners, one lane per corner (s, i) = c / nmax, c % nmax, so the lanes of a source
+// read each pair row GB[k] together: the pairs of source s are perm[soff[s] .. soff[s + 1]).
+// (One lane per source with the corner sums in registers takes twice as long: a quarter of
+// the lanes to hide the same chains of dependent loads.)
 __global__ void __launch_bounds__(kBlock) k_source_corners(int nmax, int64_t n, const int64_t* __restrict__ soff,
                                                            const int64_t* __restrict__ perm,
                                                            const double* __restrict__ GB, double* __restrict__ SC) {
@@ -461,18 +470,23 @@ __global__ void __launch_bounds__(kBlock) k_source_corners(int nmax, int64_t n, 
   reinterpret_cast<double2*>(SC)[c] = make_double2(gx, gyv);
 }
 
-// g_v[v] = sum of corner rows corners[voff[v] .. voff[v + 1]) (row index = face * nmax + slot).
-__global__ void __launch_bounds__(kBlock) k_vertex_gather(int64_t n_vertices, const int64_t* __restrict__ voff,
-                                                          const int64_t* __restrict__ corners,
-                                                          const double* __restrict__ C, double* __restrict__ g_v) {
-  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= n_vertices) return;
+// Per-vertex gather of one mesh: g_v[v] = sum of corner rows corners[voff[v] .. voff[v + 1])
+// (row index = face * nmax + slot), summed in corner order.
+struct VertexGather {
+  int64_t n_vertices;
+  const int64_t* voff;
+  const int64_t* corners;
+  const double* C;
+  double* g_v;
+};
+
+__device__ __forceinline__ void gather_vertex(const VertexGather& m, int64_t v) {
   double gx = 0.0, gy = 0.0;
-  const double2* rows = reinterpret_cast<const double2*>(C);
-  int64_t c = voff[v];
-  const int64_t end = voff[v + 1];
+  const double2* rows = reinterpret_cast<const double2*>(m.C);
+  int64_t c = m.voff[v];
+  const int64_t end = m.voff[v + 1];
   for (; c + 4 <= end; c += 4) {  // four rows in flight, summed in corner order
-    const int64_t i0 = corners[c], i1 = corners[c + 1], i2 = corners[c + 2], i3 = corners[c + 3];
+    const int64_t i0 = m.corners[c], i1 = m.corners[c + 1], i2 = m.corners[c + 2], i3 = m.corners[c + 3];
     const double2 d0 = rows[i0], d1 = rows[i1], d2 = rows[i2], d3 = rows[i3];
     gx += d0.x;
     gy += d0.y;
@@ -484,11 +498,20 @@ __global__ void __launch_bounds__(kBlock) k_vertex_gather(int64_t n_vertices, co
     gy += d3.y;
   }
   for (; c < end; ++c) {
-    const double2 d = rows[corners[c]];
+    const double2 d = rows[m.corners[c]];
     gx += d.x;
     gy += d.y;
   }
-  reinterpret_cast<double2*>(g_v)[v] = make_double2(gx, gy);
+  reinterpret_cast<double2*>(m.g_v)[v] = make_double2(gx, gy);
+}
+
+// Both meshes' vertex gathers in one launch (lanes [0, a.n) for the first, the rest for the
+// second): two short latency-bound passes share the machine instead of running one after the
+// other.
+__global__ void __launch_bounds__(kBlock) k_vertex_gather(VertexGather a, VertexGather b) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < a.n_vertices) gather_vertex(a, v);
+  else if (v - a.n_vertices < b.n_vertices) gather_vertex(b, v - a.n_vertices);
 }
 
 template <int MAXV>
@@ -963,6 +986,25 @@ int ct_barycentric_wachspress_backward(const double* points, const int64_t* face
   return CT_OK;
 }
 
+int ct_regrid_rest_targets(const ct_regrid_topology* topo, int64_t* rest_targets, int64_t* n_out, void* stream) {
+  CT_REQUIRE(topo && topo->n_tgt >= 0 && n_out, "bad arguments");
+  CT_REQUIRE(topo->n_tgt == 0 || (rest_targets && topo->tgt_offsets), "null arrays");
+  *n_out = 0;
+  if (topo->n_tgt == 0) return CT_OK;
+  hipStream_t st = as_stream(stream);
+  std::vector<int64_t> off(topo->n_tgt + 1), rest;
+  CT_HIP(hipMemcpyAsync(off.data(), topo->tgt_offsets, sizeof(int64_t) * off.size(), hipMemcpyDeviceToHost, st));
+  CT_HIP(hipStreamSynchronize(st));
+  for (int64_t t = 0; t < topo->n_tgt; ++t)
+    if (!whole_in_block(off[t], off[t + 1])) rest.push_back(t);
+  if (!rest.empty()) {
+    CT_HIP(hipMemcpyAsync(rest_targets, rest.data(), sizeof(int64_t) * rest.size(), hipMemcpyHostToDevice, st));
+    CT_HIP(hipStreamSynchronize(st));
+  }
+  *n_out = (int64_t)rest.size();
+  return CT_OK;
+}
+
 size_t ct_regrid_backward_workspace_bytes(const ct_regrid_topology* topo) {
   if (!topo) return 0;
   auto a = [](size_t b) { return (b + 255) & ~size_t(255); };
@@ -1012,14 +1054,16 @@ int ct_regrid_vertex_backward(const ct_regrid_topology* topo, const double* tgt_
 #undef CT_PAIR_BACKWARD
     CT_LAUNCH_CHECK();
   }
-  if (n_tgt > 0) {
-    const unsigned g = grid_for(n_tgt * n_max_t);
+  const int64_t* rest = topo->rest_targets;
+  const int64_t n_rt = rest ? topo->n_rest_targets : n_tgt;  // targets the corner pass visits
+  if (n_rt > 0) {
+    const unsigned g = grid_for(n_rt * n_max_t);
     if (n_max_t <= 4)
-      k_target_corners<4><<<g, kBlock, 0, st>>>(tgt_vertices, topo->tgt_faces, n_max_t, n_tgt, topo->tgt_offsets, GA,
-                                                y, gy, tgt_area, n_fields, TC);
+      k_target_corners<4><<<g, kBlock, 0, st>>>(tgt_vertices, topo->tgt_faces, n_max_t, n_rt, rest, topo->tgt_offsets,
+                                                GA, y, gy, tgt_area, n_fields, TC);
     else
-      k_target_corners<8><<<g, kBlock, 0, st>>>(tgt_vertices, topo->tgt_faces, n_max_t, n_tgt, topo->tgt_offsets, GA,
-                                                y, gy, tgt_area, n_fields, TC);
+      k_target_corners<8><<<g, kBlock, 0, st>>>(tgt_vertices, topo->tgt_faces, n_max_t, n_rt, rest, topo->tgt_offsets,
+                                                GA, y, gy, tgt_area, n_fields, TC);
     CT_LAUNCH_CHECK();
   }
   if (n_src > 0) {
@@ -1027,14 +1071,13 @@ int ct_regrid_vertex_backward(const ct_regrid_topology* topo, const double* tgt_
                                                                     topo->perm_by_src, GB, SC);
     CT_LAUNCH_CHECK();
   }
-  if (topo->n_tgt_vertices > 0) {
-    k_vertex_gather<<<grid_for(topo->n_tgt_vertices), kBlock, 0, st>>>(
-        topo->n_tgt_vertices, topo->tgt_vertex_offsets, topo->tgt_vertex_corners, TC, g_tgt_vertices);
-    CT_LAUNCH_CHECK();
-  }
-  if (topo->n_src_vertices > 0) {
-    k_vertex_gather<<<grid_for(topo->n_src_vertices), kBlock, 0, st>>>(
-        topo->n_src_vertices, topo->src_vertex_offsets, topo->src_vertex_corners, SC, g_src_vertices);
+  const int64_t n_gather = topo->n_tgt_vertices + topo->n_src_vertices;
+  if (n_gather > 0) {
+    const VertexGather tg{topo->n_tgt_vertices, topo->tgt_vertex_offsets, topo->tgt_vertex_corners, TC,
+                          g_tgt_vertices};
+    const VertexGather sg{topo->n_src_vertices, topo->src_vertex_offsets, topo->src_vertex_corners, SC,
+                          g_src_vertices};
+    k_vertex_gather<<<grid_for(n_gather), kBlock, 0, st>>>(tg, sg);
     CT_LAUNCH_CHECK();
   }
   return CT_OK;

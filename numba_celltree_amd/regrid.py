@@ -78,6 +78,13 @@ class RegridTopology:
         s.n_pairs, s.n_tgt, s.n_src = self.n_pairs, self.n_tgt, self.n_src
         s.n_tgt_vertices, s.n_src_vertices = self.n_tgt_vertices, self.n_src_vertices
         s.n_max_t, s.n_max_s = int(d_tf.shape[1]), int(self.src_faces.shape[1])
+        # the targets the vertex backward's pair pass leaves to its per-target pass
+        rest = D.empty(max(self.n_tgt, 1), torch.int64)
+        n_rest = ctypes.c_int64(0)
+        _native.call("ct_regrid_rest_targets", ctypes.byref(s), D.ptr(rest), ctypes.byref(n_rest), D.stream())
+        self.rest_targets = rest  # the first n_rest_targets entries
+        s.rest_targets = rest.data_ptr()
+        s.n_rest_targets = n_rest.value
         self._struct = s
         self._area_cache = None
 
