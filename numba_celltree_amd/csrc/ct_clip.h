@@ -130,9 +130,9 @@ __device__ __forceinline__ int sh_run(const P2 (&subj)[MA], int ls, const P2 (&c
   return len;
 }
 
-// The same clip with plain runtime indexing, for the large vertex classes (CAP > 16): the
-// select chains grow as CAP^2 per stage, and so do code size and compile time, while faces
-// of more than 8 vertices are rare.  Same operations in the same order as sh_stage.
+// The same clip with plain runtime indexing (scratch memory), for the large vertex classes
+// (CAP > 16), whose LDS rows would take more than 64 KiB per workgroup; faces of more than 8
+// vertices are rare.  Same operations in the same order as sh_stage.
 template <int MA, int MB>
 __device__ __forceinline__ double clip_area_indexed(const P2 (&subj)[MA], int ls, const P2 (&clip)[MB], int lc) {
   constexpr int CAP = MA + MB;
