@@ -87,13 +87,18 @@ def faces_to_dev(faces, n_vertices: int, allow_fill: bool = True) -> torch.Tenso
     else:
         f = host_tensor(np.asarray(faces))
     if f.numel():
-        hi, lo = int(f.max()), int(f.min())
+        # one host round trip for all three checks (a device tensor costs a synchronization
+        # per value read back)
+        lo, hi = torch.aminmax(f)
+        first3 = allow_fill and f.ndim == 2 and f.shape[1] >= 3
+        lo3 = f[:, :3].amin() if first3 else lo
+        hi, lo, lo3 = (int(v) for v in torch.stack([hi, lo, lo3]).tolist())
         if hi >= min(int(n_vertices), 2**31):
             raise ValueError(f"element index {hi} is out of bounds for {int(n_vertices)} vertices")
         if lo < (-1 if allow_fill else 0):
             what = "[-1, n_vertices)" if allow_fill else "[0, n_vertices)"
             raise ValueError(f"element indices must lie in {what}, found {lo}")
-        if allow_fill and f.ndim == 2 and f.shape[1] >= 3 and int(f[:, :3].min()) < 0:
+        if first3 and lo3 < 0:
             # polygon_length (geometry_utils.py:72-79) takes the first 3 entries as vertices
             raise ValueError("every face needs at least 3 vertices (a fill value among its first 3 entries)")
     return f.to(device=device(), dtype=torch.int32).contiguous()
@@ -103,7 +108,7 @@ def index_to_dev(x, n: int, allow_missing: bool = False) -> torch.Tensor:
     """int64 device copy of an index array into n items (-1 allowed where ``allow_missing``)."""
     t = x if isinstance(x, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(x))
     if t.numel():
-        hi, lo = int(t.max()), int(t.min())
+        lo, hi = (int(v) for v in torch.stack(torch.aminmax(t)).tolist())  # one host round trip
         if hi >= int(n) or lo < (-1 if allow_missing else 0):
             raise ValueError(f"indices must lie in [{-1 if allow_missing else 0}, {int(n)}), found [{lo}, {hi}]")
     return to_dev(t, torch.int64)
