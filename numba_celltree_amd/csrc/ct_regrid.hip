@@ -183,11 +183,15 @@ __device__ __forceinline__ void edge_range_gradient(double alpha, double beta, d
   g1 = {gA * w1 * dy, -gA * w1 * dx};
 }
 
-// A lane's target-corner accumulators in LDS (slot i at base[i * kBlock]), so the 2 * MA
-// registers they would take stay free for occupancy.
+// A lane's target-corner accumulators in LDS (slot i at base[i * kBlock], i < n = the face
+// width), so the 2 * MA registers they would take stay free for occupancy.  The gradient's
+// wrap-around edge adds 0.0 to slot la, which is past the row for a face of full width: such
+// adds (to a slot no face vertex uses) are skipped.
 struct CornerRow {
   double2* base;
+  int n;
   __device__ __forceinline__ void add(int i, double x, double y) const {
+    if (i >= n) return;
     double2 v = base[i * kBlock];
     v.x += x;
     v.y += y;
@@ -366,7 +370,7 @@ __global__ void __launch_bounds__(kBlock) k_pair_backward(
     }
     for (; f < nf; ++f) gA += gyt[f] * Fs[f];
     gA /= tarea[t];
-    const CornerRow ga{rows + lane};
+    const CornerRow ga{rows + lane, ntm};
     for (int i = 0; i < ntm; ++i) rows[i * kBlock + lane] = make_double2(0.0, 0.0);
     P2 gb[MB];
 #pragma unroll
