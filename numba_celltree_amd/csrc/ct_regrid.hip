@@ -439,6 +439,17 @@ __global__ void __launch_bounds__(kBlock) k_target_corners(const double* __restr
   reinterpret_cast<double2*>(TC)[t * nmax + slot] = make_double2(r.x, r.y);
 }
 
+// The segmented gathers below (a source's pairs, a vertex's corners) read their list in
+// batches of kGatherBatch: first the batch's indices, then the rows they address, then the
+// adds, in list order, of the positions that exist.  A position past the list's end reads
+// the list's last entry again, so every load is unconditional and in range (the loop runs
+// only for a non-empty list).  A batch costs two dependent load latencies whatever its fill;
+// a loop over the entries one at a time costs two per entry, and most lists here are 2-6
+// entries long.
+constexpr int kGatherBatch = 4;
+
+__device__ __forceinline__ int64_t clamp_pos(int64_t m, int64_t end) { return m < end ? m : end - 1; }
+
 // Source corners, one lane per corner (s, i) = c / nmax, c % nmax, so the lanes of a source
 // read each pair row GB[k] together: the pairs of source s are perm[soff[s] .. soff[s + 1]).
 // (One lane per source with the corner sums in registers takes twice as long: a quarter of
@@ -452,24 +463,20 @@ __global__ void __launch_bounds__(kBlock) k_source_corners(int nmax, int64_t n, 
   const int slot = (int)(c - s * nmax);
   double gx = 0.0, gyv = 0.0;
   const double2* gb = reinterpret_cast<const double2*>(GB) + slot;
-  int64_t m = soff[s];
   const int64_t end = soff[s + 1];
-  for (; m + 4 <= end; m += 4) {  // four rows in flight, summed in perm order
-    const int64_t k0 = perm[m], k1 = perm[m + 1], k2 = perm[m + 2], k3 = perm[m + 3];
-    const double2 d0 = gb[k0 * nmax], d1 = gb[k1 * nmax], d2 = gb[k2 * nmax], d3 = gb[k3 * nmax];
-    gx += d0.x;
-    gyv += d0.y;
-    gx += d1.x;
-    gyv += d1.y;
-    gx += d2.x;
-    gyv += d2.y;
-    gx += d3.x;
-    gyv += d3.y;
-  }
-  for (; m < end; ++m) {
-    const double2 d = gb[perm[m] * nmax];
-    gx += d.x;
-    gyv += d.y;
+  for (int64_t m = soff[s]; m < end; m += kGatherBatch) {  // summed in perm order
+    int64_t k[kGatherBatch];
+#pragma unroll
+    for (int j = 0; j < kGatherBatch; ++j) k[j] = perm[clamp_pos(m + j, end)];
+    double2 d[kGatherBatch];
+#pragma unroll
+    for (int j = 0; j < kGatherBatch; ++j) d[j] = gb[k[j] * nmax];
+#pragma unroll
+    for (int j = 0; j < kGatherBatch; ++j)
+      if (m + j < end) {
+        gx += d[j].x;
+        gyv += d[j].y;
+      }
   }
   reinterpret_cast<double2*>(SC)[c] = make_double2(gx, gyv);
 }
@@ -487,24 +494,20 @@ struct VertexGather {
 __device__ __forceinline__ void gather_vertex(const VertexGather& m, int64_t v) {
   double gx = 0.0, gy = 0.0;
   const double2* rows = reinterpret_cast<const double2*>(m.C);
-  int64_t c = m.voff[v];
   const int64_t end = m.voff[v + 1];
-  for (; c + 4 <= end; c += 4) {  // four rows in flight, summed in corner order
-    const int64_t i0 = m.corners[c], i1 = m.corners[c + 1], i2 = m.corners[c + 2], i3 = m.corners[c + 3];
-    const double2 d0 = rows[i0], d1 = rows[i1], d2 = rows[i2], d3 = rows[i3];
-    gx += d0.x;
-    gy += d0.y;
-    gx += d1.x;
-    gy += d1.y;
-    gx += d2.x;
-    gy += d2.y;
-    gx += d3.x;
-    gy += d3.y;
-  }
-  for (; c < end; ++c) {
-    const double2 d = rows[m.corners[c]];
-    gx += d.x;
-    gy += d.y;
+  for (int64_t c = m.voff[v]; c < end; c += kGatherBatch) {  // summed in corner order
+    int64_t idx[kGatherBatch];
+#pragma unroll
+    for (int j = 0; j < kGatherBatch; ++j) idx[j] = m.corners[clamp_pos(c + j, end)];
+    double2 d[kGatherBatch];
+#pragma unroll
+    for (int j = 0; j < kGatherBatch; ++j) d[j] = rows[idx[j]];
+#pragma unroll
+    for (int j = 0; j < kGatherBatch; ++j)
+      if (c + j < end) {
+        gx += d[j].x;
+        gy += d[j].y;
+      }
   }
   reinterpret_cast<double2*>(m.g_v)[v] = make_double2(gx, gy);
 }
@@ -577,13 +580,28 @@ __global__ void __launch_bounds__(kBlock) k_apply_t_rows(const int64_t* __restri
 #pragma unroll
   for (int f = 0; f < FPL; ++f) acc[f] = 0.0;
   const int64_t end = soff[s + 1];
-  for (int64_t m = soff[s]; m < end; ++m) {
-    const int64_t k = perm[m];
-    const int64_t t = src_tgt[m];
-    const double w = area[k] / tarea[t];
-    const double* row = gy + t * nf + f0;
+  for (int64_t m = soff[s]; m < end; m += kGatherBatch) {  // summed in perm order
+    int64_t k[kGatherBatch], t[kGatherBatch];
 #pragma unroll
-    for (int f = 0; f < FPL; ++f) acc[f] += w * row[f];
+    for (int j = 0; j < kGatherBatch; ++j) {
+      const int64_t mj = clamp_pos(m + j, end);
+      k[j] = perm[mj];
+      t[j] = src_tgt[mj];
+    }
+    double w[kGatherBatch], g[kGatherBatch][FPL];
+#pragma unroll
+    for (int j = 0; j < kGatherBatch; ++j) {
+      w[j] = area[k[j]] / tarea[t[j]];
+      const double* row = gy + t[j] * nf + f0;
+#pragma unroll
+      for (int f = 0; f < FPL; ++f) g[j][f] = row[f];
+    }
+#pragma unroll
+    for (int j = 0; j < kGatherBatch; ++j)
+      if (m + j < end) {
+#pragma unroll
+        for (int f = 0; f < FPL; ++f) acc[f] += w[j] * g[j][f];
+      }
   }
   double* out = gF + s * nf + f0;
 #pragma unroll
