@@ -6,8 +6,9 @@
 // a runtime position) that made the kernels bound by scratch traffic.  Here every loop over
 // a polygon is unrolled to its compile-time capacity with a `j < len` guard, so a stage reads
 // its input polygon from registers with static indices, and the runtime-indexed accesses (the
-// appended output vertices, the input's last vertex) go to the lane's row in LDS.  After each
-// stage the row is read back into registers.  The arithmetic and the branch structure are the
+// appended output vertices, the input's last vertex) go to the lane's row in LDS (its slots
+// past the tenth, used only by polygons of more than 10 vertices, in private memory).  After
+// each stage the row is read back into registers.  The arithmetic and the branch structure are the
 // reference's (sutherland_hodgman.py:56-148), operation for operation.
 #pragma once
 
@@ -29,26 +30,48 @@ __device__ __forceinline__ T get(const T (&arr)[CAP], int n) {
   return v;
 }
 
-// A lane's clip row in LDS: slot s at base[s * kBlock], lane-interleaved, so the lanes of a
-// wavefront touch 64 consecutive 16-byte words whatever slot each of them uses (no bank
-// conflicts).  A stage appends its output vertices here at runtime positions (one 16-byte
-// store each); the selects that would keep a runtime-indexed output in registers cost CAP
-// compares and 4 * CAP v_cndmask per appended vertex, and were most of the clip's
-// instructions.
+// Slots of a clip row kept in LDS.  A CU has 160 KiB of LDS: 10 slots (40 KiB per workgroup
+// of kBlock lanes) let 4 workgroups share a CU, as many as the clip kernels' registers allow,
+// where the 12 slots of the 8 + 4 vertex classes would let only 3.  Slots past these are
+// used only by clipped polygons of more than 10 vertices (a convex n-gon clipped by a convex
+// m-gon has at most n + m) and live in a small per-lane private array.
+constexpr int kClipLdsSlots = 10;
+
+// A lane's clip row of CAP slots: slot s < kLds in LDS at base[s * kBlock], lane-interleaved,
+// so the lanes of a wavefront touch 64 consecutive 16-byte words whatever slot each of them
+// uses (no bank conflicts); slot s >= kLds at extra[s - kLds].  A stage appends its output
+// vertices here at runtime positions (one 16-byte store each); the selects that would keep
+// a runtime-indexed output in registers cost CAP compares and 4 * CAP v_cndmask per appended
+// vertex, and were most of the clip's instructions.
+template <int CAP>
 struct ClipRow {
+  static constexpr int kLds = CAP < kClipLdsSlots ? CAP : kClipLdsSlots;
+  static constexpr int kExtra = CAP - kLds;
   double2* base;
-  __device__ __forceinline__ void put(int s, P2 v) const { base[s * kBlock] = double2{v.x, v.y}; }
+  double2* extra;
+  __device__ __forceinline__ void put(int s, P2 v) const {
+    const double2 d{v.x, v.y};
+    if (kExtra == 0 || s < kLds)
+      base[s * kBlock] = d;
+    else
+      extra[s - kLds] = d;
+  }
   __device__ __forceinline__ P2 get(int s) const {
-    const double2 d = base[s * kBlock];
+    double2 d;
+    if (kExtra == 0 || s < kLds)
+      d = base[s * kBlock];
+    else
+      d = extra[s - kLds];
     return P2{d.x, d.y};
   }
 };
 
-// Clip rows of capacity CAP for the kBlock lanes of a workgroup: CAP * 4 KiB of LDS.  Every
-// lane touches only its own slots, so no barrier is needed.
-#define CT_CLIP_ROWS(CAP, name)                       \
-  __shared__ double2 name##_lds[(CAP) * kBlock];      \
-  const ClipRow name { name##_lds + threadIdx.x }
+// Clip rows of capacity CAP for the kBlock lanes of a workgroup: min(CAP, 10) * 4 KiB of
+// LDS.  Every lane touches only its own slots, so no barrier is needed.
+#define CT_CLIP_ROWS(CAP, name)                                                          \
+  __shared__ double2 name##_lds[ClipRow<(CAP)>::kLds * kBlock];                          \
+  double2 name##_extra[ClipRow<(CAP)>::kExtra > 0 ? ClipRow<(CAP)>::kExtra : 1];         \
+  const ClipRow<(CAP)> name { name##_lds + threadIdx.x, name##_extra }
 
 // sutherland_hodgman.py:56-60 (strict)
 __device__ __forceinline__ bool sh_in(P2 p, P2 r, double ux, double uy) {
@@ -59,8 +82,8 @@ __device__ __forceinline__ bool sh_in(P2 p, P2 r, double ux, double uy) {
 // `row` slots [0, len)) against the clip edge r -> s; the output polygon is written to `row`
 // (over the input, which is read from registers) and its length returned.  Writes past CAP
 // are dropped (a convex n-gon clipped by a convex m-gon has at most n + m vertices).
-template <int CAP>
-__device__ __forceinline__ int sh_stage(const P2 (&in)[CAP], int len, P2 r, P2 s, const ClipRow& row) {
+template <int CAP, class Row>
+__device__ __forceinline__ int sh_stage(const P2 (&in)[CAP], int len, P2 r, P2 s, const Row& row) {
   const double ux = s.x - r.x, uy = s.y - r.y;
   const double nx = -uy, ny = ux;
   int n_out = 0;
@@ -107,9 +130,9 @@ __device__ __forceinline__ int sh_stage(const P2 (&in)[CAP], int len, P2 r, P2 s
 // Stages 0 .. upto-1 of clipping `subj` (ls vertices) by `clip` (lc vertices), skipping
 // zero-length clip edges (sutherland_hodgman.py:105-107); returns the polygon entering stage
 // `upto` in `poly`, or a length < 3 as soon as a stage leaves fewer than 3 vertices (area 0).
-template <int MA, int MB, int CAP>
+template <int MA, int MB, int CAP, class Row>
 __device__ __forceinline__ int sh_run(const P2 (&subj)[MA], int ls, const P2 (&clip)[MB], int lc, int upto,
-                                      const ClipRow& row, P2 (&poly)[CAP]) {
+                                      const Row& row, P2 (&poly)[CAP]) {
 #pragma unroll
   for (int i = 0; i < CAP; ++i) poly[i] = (i < MA) ? subj[i < MA ? i : 0] : P2{0.0, 0.0};
 #pragma unroll
@@ -131,8 +154,9 @@ __device__ __forceinline__ int sh_run(const P2 (&subj)[MA], int ls, const P2 (&c
 }
 
 // The same clip with plain runtime indexing (scratch memory), for the large vertex classes
-// (CAP > 16), whose LDS rows would take more than 64 KiB per workgroup; faces of more than 8
-// vertices are rare.  Same operations in the same order as sh_stage.
+// (CAP > 16), where most of a clip row would sit past its LDS slots anyway and the unrolled
+// loops would run to 32+ slots; faces of more than 8 vertices are rare.  Same operations in
+// the same order as sh_stage.
 template <int MA, int MB>
 __device__ __forceinline__ double clip_area_indexed(const P2 (&subj)[MA], int ls, const P2 (&clip)[MB], int lc) {
   constexpr int CAP = MA + MB;
@@ -184,21 +208,22 @@ __device__ __forceinline__ double clip_area_indexed(const P2 (&subj)[MA], int ls
   return polygon_area<CAP>(outp, n_out);
 }
 
-// sutherland_hodgman.py:84-148 polygon_polygon_clip_area.  `row` is the lane's LDS clip row
-// of clip_row_cap<MA, MB>() slots (CT_CLIP_ROWS); the indexed path for the large classes does
+// sutherland_hodgman.py:84-148 polygon_polygon_clip_area.  `row` is the lane's clip row of
+// clip_row_cap<MA, MB>() slots (CT_CLIP_ROWS); the indexed path for the large classes does
 // not use it.
 template <int MA, int MB>
 constexpr int clip_row_cap() {
   return MA + MB > 16 ? 1 : MA + MB;
 }
 
-template <int MA, int MB>
+template <int MA, int MB, class Row>
 __device__ __forceinline__ double clip_area(const P2 (&subj)[MA], int ls, const P2 (&clip)[MB], int lc,
-                                            const ClipRow& row) {
+                                            const Row& row) {
   constexpr int CAP = MA + MB;
   if constexpr (CAP > 16) {
     return clip_area_indexed<MA, MB>(subj, ls, clip, lc);
   } else {
+    static_assert(Row::kLds + Row::kExtra == CAP, "clip row of the wrong capacity");
     P2 poly[CAP];
     const int len = sh_run<MA, MB, CAP>(subj, ls, clip, lc, lc, row, poly);
     return (len < 3) ? 0.0 : polygon_area<CAP>(poly, len);

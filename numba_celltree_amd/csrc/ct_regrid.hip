@@ -17,8 +17,8 @@
 //
 // Input polygons live in registers: loops run to the compile-time capacity with a length
 // guard.  Runtime-indexed data goes to per-lane rows in LDS (the clip's output polygon,
-// ct_clip.h; the adjoint's target-corner sums) or through select chains, so no kernel here
-// touches scratch memory.
+// ct_clip.h, whose slots past the tenth are private memory that only polygons of more than 10
+// vertices reach; the adjoint's target-corner sums) or through select chains.
 #include <vector>
 
 #include "ct_clip.h"

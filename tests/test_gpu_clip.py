@@ -56,7 +56,17 @@ def _pairs(seed):
     col = np.array([[0.0, 0.0], [0.5, 0.0], [1.0, 0.0], [1.0, 1.0], [0.0, 1.0]])
     out.append((col, sq + [0.3, -0.2]))
     out.append((sq + [0.3, -0.2], col))
+    out.extend(_full_rows())
     return out
+
+
+def _full_rows():
+    """Pairs whose clipped polygon fills the clip row past its LDS slots: a regular octagon and
+    a square that cuts off every other corner (12 vertices), both ways round, and two octagons
+    turned by 22.5 degrees (16 vertices)."""
+    octagon = _ngon(8, 0.0, 0.0, 1.0)
+    square = np.array([[-0.95, -0.95], [0.95, -0.95], [0.95, 0.95], [-0.95, 0.95]])
+    return [(octagon, square), (square, octagon), (octagon, _ngon(8, 0.0, 0.0, 1.0, np.pi / 8))]
 
 
 def _as_mesh(polys, width):
@@ -94,6 +104,26 @@ def test_area_of_intersection_matches_oracle(seed):
                                 np.array_equal(pairs[k][1], pairs[k][0] + [1.0, 0.0]) for k in sel])
         assert shared_edge.sum() == 1 and got[shared_edge][0] == 0.0
     assert len(pairs) > 100
+
+
+def test_intersect_faces_full_rows_match_oracle():
+    """The same 12- and 16-vertex intersections through intersect_faces (the pair kernel with
+    its SAT pre-test), several copies of each polygon shifted so every pair class holds some."""
+    import numba_celltree_amd as M
+
+    pairs = _full_rows()
+    shifts = [np.array([3.0 * i, 0.0]) for i in range(4)]  # copies 3 apart overlap only within a copy
+    for wa, wb in ((8, 4), (4, 8), (8, 8)):
+        sel = [(a, b) for a, b in pairs if len(a) <= wa and len(b) <= wb]
+        va, fa = _as_mesh([a + d for a, _ in sel for d in shifts], wa)
+        vb, fb = _as_mesh([b + d for _, b in sel for d in shifts], wb)
+        gi, gj, ga = M.CellTree2d(va, fa, -1).intersect_faces(vb, fb, -1)
+        oi, oj, oa = R.OracleCellTree2d(va, fa, -1).intersect_faces(vb, fb, -1)
+        print(f"[clip] intersect_faces widths {wa}x{wb}: {len(gi)} pairs, max diff "
+              f"{np.abs(ga - oa).max():.3e}", file=sys.stderr, flush=True)
+        assert np.array_equal(gi, oi) and np.array_equal(gj, oj)
+        assert len(gi) >= len(sel) * len(shifts)
+        np.testing.assert_allclose(ga, oa, rtol=1e-12, atol=0.0)  # the parity tests' bar
 
 
 def test_box_area_of_intersection_matches_oracle():
