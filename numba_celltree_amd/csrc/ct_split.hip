@@ -296,9 +296,11 @@ __global__ void __launch_bounds__(kBlock) k_boxes_sub(TreeView t, const double* 
 
 // Box rows of the subqueries whose hits all fit the record, copied from it.  One wavefront
 // per workgroup: it reads its lanes' record slots coalesced (slot-major: slot h of lane j at
-// hits[h * n + j]) into LDS, 32 slots at a time, then writes the rows of two lanes per step as
-// two contiguous runs of up to 32, instead of 64 scattered 8-byte stores per instruction.
-constexpr int kReplayRows = 32;
+// hits[h * n + j]) into LDS, kReplayRows slots at a time, then writes the rows of
+// 64 / kReplayRows lanes per step as contiguous runs of up to kReplayRows, instead of 64
+// scattered 8-byte stores per instruction.
+constexpr int kReplayRows = 16;
+constexpr int kReplayLanes = 64 / kReplayRows;  // lanes whose rows one step writes
 
 __global__ void __launch_bounds__(64) k_boxes_replay(const ct_subquery* __restrict__ subs, int64_t n,
                                                      const int64_t* __restrict__ query_offsets,
@@ -325,14 +327,15 @@ __global__ void __launch_bounds__(64) k_boxes_replay(const ct_subquery* __restri
       buf[c][lane] = h < total ? hits[h * n + i] : 0;
     }
     __syncthreads();
-    // two lanes' rows per step: lanes 0-31 write lane l's, lanes 32-63 lane l + 1's
-    const int half = lane >> 5, r = lane & 31;
+    // kReplayLanes lanes' rows per step: lanes [p * kReplayRows, (p + 1) * kReplayRows)
+    // write lane l + p's
+    const int part = lane / kReplayRows, r = lane % kReplayRows;
     const int64_t h = h0 + r;
-    for (int l = 0; l < 64; l += 2) {
-      const int64_t tl = __shfl(total, l + half);
-      const int64_t ol = __shfl(o, l + half), ql = __shfl(q, l + half);
+    for (int l = 0; l < 64; l += kReplayLanes) {
+      const int64_t tl = __shfl(total, l + part);
+      const int64_t ol = __shfl(o, l + part), ql = __shfl(q, l + part);
       if (h < tl) {
-        jj[ol + h] = buf[r][l + half];
+        jj[ol + h] = buf[r][l + part];
         ii[ol + h] = ql;
       }
     }
