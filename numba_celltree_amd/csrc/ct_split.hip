@@ -184,8 +184,9 @@ __global__ void __launch_bounds__(kBlock) k_split(TreeView t, int kind, const do
                                                   const int32_t* __restrict__ order, int64_t n, SplitParams p,
                                                   int64_t* __restrict__ parts, const int64_t* __restrict__ part_offsets,
                                                   ct_subquery* __restrict__ subs, int32_t* __restrict__ keep) {
-  // per lane: two frontier buffers (LDS, lane-interleaved)
-  __shared__ int lds[2 * kMaxParts * kBlock];
+  // per lane: two frontier buffers of p.max_parts entries (LDS, lane-interleaved), sized at
+  // launch: 2 * max_parts * kBlock ints (32 KiB at 16 parts, 5 workgroups per CU; 16 KiB at 8)
+  extern __shared__ int lds[];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int64_t q = query_index(order, i);
@@ -195,7 +196,7 @@ __global__ void __launch_bounds__(kBlock) k_split(TreeView t, int kind, const do
     return;
   }
   int* a = lds + threadIdx.x;
-  int* b = a + kMaxParts * kBlock;
+  int* b = a + p.max_parts * kBlock;
   const SplitQuery s = load_split_query<FACES>(t, kind, qs, q);
   int* fr;
   const int m = frontier<FACES>(t, s, p, a, b, &fr);
@@ -738,10 +739,13 @@ int ct_split_queries(const ct_tree* tree, int32_t kind, const double* queries, i
   const TreeView t = view(tree);
   hipStream_t st = as_stream(stream);
   const int k = kind == 1 ? 1 : 2;  // box or segment
+  const size_t lds = 2 * (size_t)max_parts * kBlock * sizeof(int);
   if (kind == 2)
-    k_split<true><<<grid_for(n), kBlock, 0, st>>>(t, k, queries, order, n, p, parts, part_offsets, subs, frontier_buf);
+    k_split<true><<<grid_for(n), kBlock, lds, st>>>(t, k, queries, order, n, p, parts, part_offsets, subs,
+                                                    frontier_buf);
   else
-    k_split<false><<<grid_for(n), kBlock, 0, st>>>(t, k, queries, order, n, p, parts, part_offsets, subs, frontier_buf);
+    k_split<false><<<grid_for(n), kBlock, lds, st>>>(t, k, queries, order, n, p, parts, part_offsets, subs,
+                                                     frontier_buf);
   CT_LAUNCH_CHECK();
   return CT_OK;
 }
